@@ -1,0 +1,188 @@
+// glc_access.hip -- random-access kernels of libglc.
+//
+// One lane per query, grid-stride over all q queries (the reference caps the
+// grid and silently drops queries past it, api/G-LeCo.cuh:655-662, :741,
+// :1308, :1973).  Partition lookup:
+//   generic  tile_part[pos >> 10] gives the last partition starting at or
+//            before the 1024-element tile; a binary search between
+//            tile_part[t] and tile_part[t+1] finishes it (usually 0 steps),
+//            replacing the ~log2(P) dependent global probes of
+//            decompression/RandomAccessKernels.cuh:82-98;
+//   fixed    pos / S (decompression/FixedPartitionKernels.cuh:76-77), with S
+//            validated against the column on the host;
+//   blob     binary search over the blob's start/end arrays (direct access
+//            handle, RandomAccessKernels.cuh:174-333).
+// Positions outside [0, N) or in a gap decode to 0 (RandomAccessKernels.cuh:100-103).
+#include <hip/hip_runtime.h>
+
+#include "glc_device.hpp"
+#include "glc_internal.hpp"
+
+namespace glc {
+namespace {
+
+constexpr int kThreads = 256;
+
+struct RaArgs {
+    const Desc* __restrict__ desc;
+    const int32_t* __restrict__ tile_part;
+    const uint32_t* __restrict__ words;
+    const int64_t* __restrict__ plain;
+    const double* __restrict__ params;
+    const void* __restrict__ pos;
+    void* __restrict__ out;
+    int64_t q;
+    int64_t n;
+    int64_t P;
+    int64_t n_index;
+    int32_t fixed;  // S for the fixed lookup, 0 = generic
+};
+
+template <class T, bool POS64, bool PLAIN>
+__global__ __launch_bounds__(kThreads) void k_random_access(RaArgs a) {
+    T* __restrict__ out = static_cast<T*>(a.out);
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * kThreads;
+    for (int64_t j = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x; j < a.q; j += stride) {
+        const int64_t e = POS64 ? static_cast<const int64_t*>(a.pos)[j]
+                                : static_cast<int64_t>(static_cast<const int32_t*>(a.pos)[j]);
+        T v = T(0);
+        if (e >= 0 && e < a.n) {
+            int64_t p;
+            if (a.fixed > 0) {
+                p = e / a.fixed;
+            } else {
+                const int64_t t = e >> kIndexShift;
+                p = a.tile_part[t];
+                int64_t hi = a.tile_part[t + 1];
+                while (p < hi) {
+                    const int64_t m = (p + hi + 1) >> 1;
+                    if (a.desc[m].start <= e) p = m; else hi = m - 1;
+                }
+            }
+            const Desc d = a.desc[p];
+            const int64_t k = e - d.start;
+            if (k >= 0 && k < d.len) {
+                const int32_t type = desc_type(d.packed);
+                const int b = desc_bits(d.packed);
+                const int32_t i = static_cast<int32_t>(k);
+                if constexpr (PLAIN) {
+                    const int64_t pl = a.plain[e];
+                    if (type == MODEL_DIRECT_COPY) v = static_cast<T>(pl);
+                    else if (type == MODEL_POLY2)
+                        v = apply_delta<T>(dec_pred_poly<T>(type, d.theta0, d.theta1, a.params[4 * p + 2], i), pl);
+                    else v = apply_delta<T>(dec_pred<T>(d.theta0, d.theta1, i), pl);
+                } else {
+                    const uint64_t raw = (b > 0) ? extract_raw_g(a.words, desc_off(d.packed) + k * b, b) : 0;
+                    if (type == MODEL_DIRECT_COPY) v = static_cast<T>(raw);
+                    else if (type == MODEL_POLY2)
+                        v = apply_delta<T>(dec_pred_poly<T>(type, d.theta0, d.theta1, a.params[4 * p + 2], i),
+                                           sign_extend(raw, b));
+                    else v = apply_delta<T>(dec_pred<T>(d.theta0, d.theta1, i), sign_extend(raw, b));
+                }
+            }
+        }
+        out[j] = v;
+    }
+}
+
+// Direct access on a device-resident blob: the reference's own lookup
+// (binary search over start/end, RandomAccessKernels.cuh:193-227).
+template <class T>
+__global__ __launch_bounds__(kThreads) void k_direct_access(BlobView bv, const int32_t* __restrict__ pos, int64_t q,
+                                                            T* __restrict__ out) {
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * kThreads;
+    for (int64_t j = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x; j < q; j += stride) {
+        const int64_t e = pos[j];
+        int64_t l = 0, r = bv.P - 1, f = -1;
+        while (l <= r) {
+            const int64_t m = (l + r) >> 1;
+            const int32_t s = bv.start[m], en = bv.end[m];
+            if (e >= s && e < en) { f = m; break; }
+            if (e >= en) l = m + 1; else r = m - 1;
+        }
+        T v = T(0);
+        if (f >= 0) {
+            const int32_t type = bv.type[f];
+            const int b = bv.bits[f];
+            const int64_t k = e - bv.start[f];
+            const uint64_t raw = (b > 0 && b <= 64) ? extract_raw_g(bv.words, bv.off[f] + k * b, b) : 0;
+            const int32_t i = static_cast<int32_t>(k);
+            if (type == MODEL_DIRECT_COPY) v = static_cast<T>(raw);
+            else
+                v = apply_delta<T>(dec_pred_poly<T>(type, bv.params[4 * f], bv.params[4 * f + 1], bv.params[4 * f + 2], i),
+                                   sign_extend(raw, b));
+        }
+        out[j] = v;
+    }
+}
+
+template <class F> int dispatch_dtype(int dtype, F&& f) {
+    switch (dtype) {
+        case GLC_I32: return f(int32_t());
+        case GLC_U32: return f(uint32_t());
+        case GLC_I64: return f(int64_t());
+        case GLC_U64: return f(uint64_t());
+        default: return fail(GLC_ERR_INVALID_ARG, "bad dtype");
+    }
+}
+
+unsigned ra_grid(int64_t q) {
+    int64_t g = (q + kThreads - 1) / kThreads;
+    const int64_t cap = 256LL * 64;  // plenty of waves to cover HBM latency; grid-stride beyond
+    if (g > cap) g = cap;
+    if (g < 1) g = 1;
+    return static_cast<unsigned>(g);
+}
+
+}  // namespace
+
+int launch_random_access(const glc_column* c, const void* d_pos, int pos64, int64_t q, void* d_out, int mode,
+                         int32_t fixed_size, hipStream_t s) {
+    if (q == 0) return GLC_OK;
+    const bool plain = (mode == GLC_RA_PREUNPACKED);
+    if (mode != GLC_RA_GENERIC && mode != GLC_RA_PREUNPACKED) return fail(GLC_ERR_INVALID_ARG, "bad random-access mode");
+    if (plain && !c->d_plain)
+        return fail(GLC_ERR_UNSUPPORTED, "no pre-unpacked deltas (call glc_preunpack or deserialize with preunpack)");
+    if (!plain && !c->d_words && c->total_bits > 0)
+        return fail(GLC_ERR_UNSUPPORTED, "bit-packed payload was released by glc_preunpack(free_packed=1)");
+    RaArgs a;
+    a.desc = c->d_desc;
+    a.tile_part = c->d_tile_part;
+    a.words = c->d_words;
+    a.plain = c->d_plain;
+    a.params = c->d_params;
+    a.pos = d_pos;
+    a.out = d_out;
+    a.q = q;
+    a.n = c->n;
+    a.P = c->P;
+    a.n_index = c->n_index;
+    a.fixed = fixed_size;
+    if (c->P == 0) a.n = 0;  // every query decodes to 0
+    return dispatch_dtype(c->dtype, [&](auto t) {
+        using T = decltype(t);
+        const dim3 g(ra_grid(q));
+        if (pos64) {
+            if (plain) hipLaunchKernelGGL((k_random_access<T, true, true>), g, dim3(kThreads), 0, s, a);
+            else hipLaunchKernelGGL((k_random_access<T, true, false>), g, dim3(kThreads), 0, s, a);
+        } else {
+            if (plain) hipLaunchKernelGGL((k_random_access<T, false, true>), g, dim3(kThreads), 0, s, a);
+            else hipLaunchKernelGGL((k_random_access<T, false, false>), g, dim3(kThreads), 0, s, a);
+        }
+        GLC_HIP(hipGetLastError());
+        return GLC_OK;
+    });
+}
+
+int launch_direct_access(int dtype, const BlobView& v, const int32_t* d_pos, int64_t q, void* d_out, hipStream_t s) {
+    if (q == 0) return GLC_OK;
+    return dispatch_dtype(dtype, [&](auto t) {
+        using T = decltype(t);
+        hipLaunchKernelGGL(k_direct_access<T>, dim3(ra_grid(q)), dim3(kThreads), 0, s, v, d_pos, q,
+                           static_cast<T*>(d_out));
+        GLC_HIP(hipGetLastError());
+        return GLC_OK;
+    });
+}
+
+}  // namespace glc

@@ -1,0 +1,522 @@
+// glc_compress.hip -- compress-side gfx950 kernels of libglc.
+//
+//   k_fit         per-partition overflow check + FP64 linear fit + max |delta|
+//                 -> bit width (compression/CompressionKernels.cuh:46-218)
+//   k_scan        exclusive scan of len*bits -> bit offsets
+//                 (replaces the O(P^2) setBitOffsetsKernel, :222-238)
+//   k_build_index 32-byte decode descriptors + 1024-element tile index
+//   k_pack        bit packer: one workgroup per 2048-element tile, deltas ORed
+//                 into an LDS window, interior words stored once, only the two
+//                 edge words of a tile use a global atomicOr
+//                 (replaces the per-element atomicOr packer, :243-372)
+//   k_v6_*        GPUVariableLengthPartitionerV6 (compression/GpuPartitioner.cuh:34-214,
+//                 compression/PartitioningKernels.cuh:648-744) without atomics
+//                 or sorts: variance per block, radix selection of the three
+//                 quartiles, counted prefix sum to emit partitions in order.
+#include <hip/hip_runtime.h>
+
+#include "glc_device.hpp"
+#include "glc_internal.hpp"
+
+namespace glc {
+namespace {
+
+constexpr int kFitThreads = 256;
+
+// ---------------------------------------------------------------------------
+// Fit.  One workgroup per partition (grid-stride over partitions).
+// The four sums are accumulated by the first `bs` threads over i = t, t+bs, ...
+// and reduced by the shared-memory tree s = bs/2, bs/4, ..., 1 exactly as the
+// reference does (:113-137), so theta0/theta1 are bit-identical for the same
+// contraction form (fma in sum_xx, sum_xy, det, numerator and theta0).
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ __launch_bounds__(kFitThreads) void k_fit(const T* __restrict__ v, const int32_t* __restrict__ st,
+                                                     const int32_t* __restrict__ en, int32_t* __restrict__ type,
+                                                     double* __restrict__ params, int32_t* __restrict__ bits,
+                                                     int64_t* __restrict__ err, int64_t P, int bs) {
+    __shared__ double s_sum[4][kFitThreads];
+    __shared__ double s_theta[2];
+    __shared__ int64_t s_max[kFitThreads / 64];
+    const int tid = threadIdx.x;
+    for (int64_t p = blockIdx.x; p < P; p += gridDim.x) {
+        const int32_t s0 = st[p];
+        const int32_t n = en[p] - s0;
+        if (n <= 0) continue;  // :62, entry keeps its initial values
+        const T* __restrict__ x = v + s0;
+        double a = 0.0, b = 0.0, c = 0.0, d = 0.0;
+        int ovf = 0;
+        if (tid < bs) {
+            for (int32_t i = tid; i < n; i += bs) {
+                const T val = x[i];
+                ovf |= might_overflow_double<T>(val) ? 1 : 0;
+                const double xi = static_cast<double>(i);
+                const double yi = static_cast<double>(val);
+                a = a + xi;
+                b = b + yi;
+                c = __builtin_fma(xi, xi, c);
+                d = __builtin_fma(xi, yi, d);
+            }
+        }
+        if (__syncthreads_or(ovf)) {  // :95-106 DIRECT_COPY
+            if (tid == 0) {
+                type[p] = MODEL_DIRECT_COPY;
+                params[4 * p] = 0.0; params[4 * p + 1] = 0.0; params[4 * p + 2] = 0.0; params[4 * p + 3] = 0.0;
+                err[p] = 0;
+                bits[p] = static_cast<int32_t>(sizeof(T) * 8);
+            }
+            __syncthreads();
+            continue;
+        }
+        s_sum[0][tid] = a; s_sum[1][tid] = b; s_sum[2][tid] = c; s_sum[3][tid] = d;
+        __syncthreads();
+        for (int h = bs >> 1; h > 0; h >>= 1) {
+            if (tid < h) {
+                s_sum[0][tid] = s_sum[0][tid] + s_sum[0][tid + h];
+                s_sum[1][tid] = s_sum[1][tid] + s_sum[1][tid + h];
+                s_sum[2][tid] = s_sum[2][tid] + s_sum[2][tid + h];
+                s_sum[3][tid] = s_sum[3][tid] + s_sum[3][tid + h];
+            }
+            __syncthreads();
+        }
+        if (tid == 0) {
+            const double dn = static_cast<double>(n);
+            const double Sx = s_sum[0][0], Sy = s_sum[1][0], Sxx = s_sum[2][0], Sxy = s_sum[3][0];
+            const double det = __builtin_fma(dn, Sxx, -(Sx * Sx));
+            double t0, t1;
+            if (__builtin_fabs(det) > 1e-10) {
+                t1 = __builtin_fma(dn, Sxy, -(Sx * Sy)) / det;
+                t0 = __builtin_fma(-t1, Sx, Sy) / dn;
+            } else {
+                t1 = 0.0;
+                t0 = Sy / dn;
+            }
+            s_theta[0] = t0; s_theta[1] = t1;
+            type[p] = MODEL_LINEAR;
+            params[4 * p] = t0; params[4 * p + 1] = t1; params[4 * p + 2] = 0.0; params[4 * p + 3] = 0.0;
+        }
+        __syncthreads();
+        const double t0 = s_theta[0], t1 = s_theta[1];
+        int64_t mx = 0;
+        for (int32_t i = tid; i < n; i += kFitThreads) {
+            const int64_t dl = calc_delta<T>(x[i], enc_pred<T>(t0, t1, i));
+            const int64_t ab = dl < 0 ? static_cast<int64_t>(0ULL - static_cast<uint64_t>(dl)) : dl;
+            mx = ab > mx ? ab : mx;  // -LLONG_MIN stays negative and is ignored, as in :171-174
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            const int64_t y = __shfl_xor(mx, o, 64);
+            mx = y > mx ? y : mx;
+        }
+        if ((tid & 63) == 0) s_max[tid >> 6] = mx;
+        __syncthreads();
+        if (tid == 0) {
+            int64_t m = s_max[0];
+            for (int w = 1; w < kFitThreads / 64; ++w) m = s_max[w] > m ? s_max[w] : m;
+            err[p] = m;
+            int32_t bb = 0;
+            if (m > 0) {
+                bb = 64 - __builtin_clzll(static_cast<unsigned long long>(m)) + 1;
+                bb = bb > 64 ? 64 : bb;
+            }
+            bits[p] = bb;
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Single-workgroup exclusive scan of int64 values produced by a functor.
+// ---------------------------------------------------------------------------
+constexpr int kScanThreads = 1024;
+
+struct LenBits {
+    const int32_t* st;
+    const int32_t* en;
+    const int32_t* bits;
+    __device__ int64_t operator()(int64_t p) const {
+        return static_cast<int64_t>(en[p] - st[p]) * static_cast<int64_t>(bits[p]);
+    }
+};
+
+template <class F>
+__global__ __launch_bounds__(kScanThreads) void k_scan(F f, int64_t P, int64_t* __restrict__ out,
+                                                       int64_t* __restrict__ total) {
+    __shared__ int64_t s[kScanThreads];
+    const int tid = threadIdx.x;
+    const int64_t chunk = (P + kScanThreads - 1) / kScanThreads;
+    const int64_t b0 = tid * chunk;
+    const int64_t b1 = (b0 + chunk < P) ? b0 + chunk : P;
+    int64_t local = 0;
+    for (int64_t p = b0; p < b1; ++p) local += f(p);
+    s[tid] = local;
+    __syncthreads();
+    for (int o = 1; o < kScanThreads; o <<= 1) {
+        int64_t y = (tid >= o) ? s[tid - o] : 0;
+        __syncthreads();
+        s[tid] += y;
+        __syncthreads();
+    }
+    int64_t acc = s[tid] - local;  // exclusive prefix of this chunk
+    for (int64_t p = b0; p < b1; ++p) {
+        out[p] = acc;
+        acc += f(p);
+    }
+    if (tid == kScanThreads - 1) *total = s[tid];
+}
+
+// ---------------------------------------------------------------------------
+// Decode index: Desc[P] + tile_part[] + validation summary.
+// ---------------------------------------------------------------------------
+__global__ void k_build_index(const int32_t* __restrict__ st, const int32_t* __restrict__ en,
+                              const int32_t* __restrict__ type, const double* __restrict__ params,
+                              const int32_t* __restrict__ bits, const int64_t* __restrict__ off, int64_t P,
+                              int64_t n, Desc* __restrict__ desc, int32_t* __restrict__ tile_part, int64_t n_index,
+                              IndexSummary* __restrict__ sum) {
+    const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    const int32_t s = st[p], e = en[p], b = bits[p], ty = type[p];
+    const int32_t len = e > s ? e - s : 0;
+    Desc d;
+    d.theta0 = params[4 * p];
+    d.theta1 = params[4 * p + 1];
+    d.packed = desc_pack(off[p], b, ty);
+    d.start = s;
+    d.len = len;
+    desc[p] = d;
+    // validation (sorted, non-overlapping, inside [0, n))
+    int bad = (s < 0 || e < s || static_cast<int64_t>(e) > n) ? 1 : 0;
+    const int32_t s_next = (p + 1 < P) ? st[p + 1] : 0x7FFFFFFF;
+    if (p + 1 < P && e > s_next) bad = 1;
+    if (p + 1 < P && s_next < s) bad = 1;
+    if (bad) atomicOr(&sum->malformed, 1);
+    if (b < 0 || b > 64) atomicOr(&sum->bad_bits, 1);
+    if (ty == MODEL_POLY2) atomicOr(&sum->has_poly, 1);
+    atomicMax(&sum->max_bits, b);
+    const int64_t endbit = off[p] + static_cast<int64_t>(len) * b;
+    atomicMax(reinterpret_cast<unsigned long long*>(&sum->total_bits), static_cast<unsigned long long>(endbit));
+    // fixed-size detection against S = len of partition 0
+    const int64_t S = static_cast<int64_t>(en[0]) - st[0];
+    const int64_t fs = S * p;
+    const int64_t fe = (fs + S < n) ? fs + S : n;
+    if (S <= 0 || s != fs || e != fe || (p == P - 1 && e != n)) atomicOr(&sum->not_fixed, 1);
+    // tile_part[t] = last partition with start <= t*1024, for t in [ceil(s/1024), ceil(s_next/1024))
+    if (bad) return;
+    int64_t t0 = (p == 0) ? 0 : (static_cast<int64_t>(s) + kIndexTile - 1) >> kIndexShift;
+    int64_t t1 = (p + 1 < P) ? (static_cast<int64_t>(s_next) + kIndexTile - 1) >> kIndexShift : n_index;
+    if (t1 > n_index) t1 = n_index;
+    for (int64_t t = t0; t < t1; ++t) tile_part[t] = static_cast<int32_t>(p);
+}
+
+// ---------------------------------------------------------------------------
+// Pack.  One workgroup per 2048-element tile.
+// ---------------------------------------------------------------------------
+constexpr int kPackThreads = 256;
+constexpr int kPackTile = 2048;
+constexpr int kPackWords = kPackTile * 2 + 8;
+
+// bit position of element e's first bit, or of the next stored bit when e is
+// in a gap / past the end of the last partition with start <= e.
+__device__ __forceinline__ int64_t bitpos_lb(const Desc* __restrict__ desc, int64_t plo, int64_t phi, int64_t e) {
+    // last partition in [plo, phi] with start <= e
+    int64_t lo = plo, hi = phi;
+    while (lo < hi) {
+        int64_t m = (lo + hi + 1) >> 1;
+        if (desc[m].start <= e) lo = m; else hi = m - 1;
+    }
+    const Desc d = desc[lo];
+    if (e < d.start) return desc_off(d.packed);
+    int64_t k = e - d.start;
+    if (k > d.len) k = d.len;
+    return desc_off(d.packed) + k * desc_bits(d.packed);
+}
+
+template <class T>
+__global__ __launch_bounds__(kPackThreads) void k_pack(const T* __restrict__ v, const Desc* __restrict__ desc,
+                                                       const int32_t* __restrict__ tile_part, int64_t n,
+                                                       uint32_t* __restrict__ words) {
+    __shared__ uint32_t s_w[kPackWords];
+    __shared__ int64_t s_w0, s_w1;
+    const int tid = threadIdx.x;
+    const int64_t e0 = static_cast<int64_t>(blockIdx.x) * kPackTile;
+    if (e0 >= n) return;
+    const int64_t e1 = (e0 + kPackTile < n) ? e0 + kPackTile : n;
+    const int64_t plo = tile_part[e0 >> kIndexShift];
+    const int64_t phi = tile_part[(e1 + kIndexTile - 1) >> kIndexShift];
+    if (tid == 0) {
+        s_w0 = bitpos_lb(desc, plo, phi, e0);
+        s_w1 = bitpos_lb(desc, plo, phi, e1 - 1);
+    }
+    for (int i = tid; i < kPackWords; i += kPackThreads) s_w[i] = 0u;
+    __syncthreads();
+    const int64_t wbase = s_w0 >> 5;  // first word touched by this tile
+    for (int64_t e = e0 + tid; e < e1; e += kPackThreads) {
+        int64_t lo = plo, hi = phi;
+        while (lo < hi) {
+            int64_t m = (lo + hi + 1) >> 1;
+            if (desc[m].start <= e) lo = m; else hi = m - 1;
+        }
+        const Desc d = desc[lo];
+        if (e < d.start || e >= static_cast<int64_t>(d.start) + d.len) continue;
+        const int b = desc_bits(d.packed);
+        if (b == 0) continue;
+        const int32_t i = static_cast<int32_t>(e - d.start);
+        uint64_t code;
+        if (desc_type(d.packed) == MODEL_DIRECT_COPY) {
+            code = static_cast<uint64_t>(v[e]);
+        } else {
+            code = static_cast<uint64_t>(calc_delta<T>(v[e], enc_pred<T>(d.theta0, d.theta1, i)));
+        }
+        if (b < 64) code &= (1ULL << b) - 1ULL;
+        const int64_t pos = desc_off(d.packed) + static_cast<int64_t>(i) * b;
+        const uint32_t rel = static_cast<uint32_t>(pos - (wbase << 5));
+        const uint32_t wi = rel >> 5, sh = rel & 31;
+        atomicOr(&s_w[wi], static_cast<uint32_t>(code << sh));
+        if (sh + b > 32) atomicOr(&s_w[wi + 1], static_cast<uint32_t>(code >> (32 - sh)));
+        if (sh + b > 64) atomicOr(&s_w[wi + 2], static_cast<uint32_t>(code >> (64 - sh)));
+    }
+    __syncthreads();
+    // words [wbase, wlast]; the first and last may be shared with neighbours
+    const int64_t wlast = (s_w1 >= 0) ? (s_w1 + 64) >> 5 : wbase;  // conservative: covers the last element
+    const int64_t nw = wlast - wbase + 1;
+    for (int64_t k = tid; k < nw && k < kPackWords; k += kPackThreads) {
+        const uint32_t val = s_w[k];
+        if (k == 0 || k >= nw - 3) {
+            if (val) atomicOr(&words[wbase + k], val);
+        } else {
+            words[wbase + k] = val;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// V6 partitioner
+// ---------------------------------------------------------------------------
+constexpr int kV6Threads = 128;  // compression/GpuPartitioner.cuh:53
+
+template <class T>
+__global__ __launch_bounds__(kV6Threads) void k_v6_var(const T* __restrict__ v, int64_t n, int64_t vbs,
+                                                       int64_t nvb, float* __restrict__ var) {
+    __shared__ float s_stat[4];
+    const int tid = threadIdx.x;
+    for (int64_t blk = blockIdx.x; blk < nvb; blk += gridDim.x) {
+        const int64_t s = blk * vbs;
+        const int64_t e = (s + vbs < n) ? s + vbs : n;
+        const double dn = static_cast<double>(e - s);
+        double sum = 0.0, ssq = 0.0, c1 = 0.0, c2 = 0.0;
+        for (int64_t i = s + tid; i < e; i += kV6Threads) {  // Kahan, PartitioningKernels.cuh:669-682
+            const double val = static_cast<double>(v[i]);
+            const double y1 = val - c1;
+            const double t1 = sum + y1;
+            c1 = (t1 - sum) - y1;
+            sum = t1;
+            const double y2 = __builtin_fma(val, val, -c2);
+            const double t2 = ssq + y2;
+            c2 = (t2 - ssq) - y2;
+            ssq = t2;
+        }
+        for (int o = 16; o > 0; o >>= 1) {  // 32-lane tree of MathHelpers.cuh:91-95
+            sum = sum + __shfl_down(sum, o, 32);
+            ssq = ssq + __shfl_down(ssq, o, 32);
+        }
+        if ((tid & 31) == 0) {
+            const double q = sum / dn;
+            s_stat[tid >> 5] = static_cast<float>(__builtin_fma(-q, q, ssq / dn));
+        }
+        __syncthreads();
+        if (tid == 0) {
+            float acc = 0.0f;
+            for (int w = 0; w < 4; ++w) acc = acc + s_stat[w];
+            var[blk] = acc;
+        }
+        __syncthreads();
+    }
+}
+
+__device__ __forceinline__ uint32_t float_key(float f) {
+    const uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float key_float(uint32_t k) {
+    const uint32_t u = (k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k;
+    return __uint_as_float(u);
+}
+
+// Order statistics sorted[nvb/4], sorted[nvb/2], sorted[3nvb/4] by 4-pass
+// radix selection (one workgroup; the reference sorts with thrust::sort).
+__global__ __launch_bounds__(1024) void k_v6_select(const float* __restrict__ var, int64_t nvb,
+                                                    float* __restrict__ thr) {
+    __shared__ uint32_t hist[256];
+    __shared__ uint32_t s_prefix;
+    __shared__ int64_t s_rank;
+    const int tid = threadIdx.x;
+    for (int q = 0; q < 3; ++q) {
+        const int64_t rank0 = (q == 0) ? nvb / 4 : (q == 1) ? nvb / 2 : 3 * nvb / 4;
+        if (tid == 0) { s_prefix = 0; s_rank = rank0; }
+        __syncthreads();
+        for (int pass = 0; pass < 4; ++pass) {
+            const int shift = 24 - 8 * pass;
+            const uint32_t hmask = (pass == 0) ? 0u : (0xFFFFFFFFu << (32 - 8 * pass));
+            if (tid < 256) hist[tid] = 0;
+            __syncthreads();
+            const uint32_t prefix = s_prefix;
+            for (int64_t i = tid; i < nvb; i += 1024) {
+                const uint32_t k = float_key(var[i]);
+                if ((k & hmask) == prefix) atomicAdd(&hist[(k >> shift) & 0xFF], 1u);
+            }
+            __syncthreads();
+            if (tid == 0) {
+                int64_t r = s_rank;
+                uint32_t dgt = 0;
+                for (; dgt < 256; ++dgt) {
+                    if (r < hist[dgt]) break;
+                    r -= hist[dgt];
+                }
+                s_rank = r;
+                s_prefix = prefix | (dgt << shift);
+            }
+            __syncthreads();
+        }
+        if (tid == 0) thr[q] = key_float(s_prefix);
+        __syncthreads();
+    }
+}
+
+struct V6Count {
+    const float* var;
+    const float* thr;
+    int64_t vbs, n;
+    int32_t base;
+    __device__ int64_t size_of(int64_t b) const {  // PartitioningKernels.cuh:722-731
+        const float x = var[b];
+        return x < thr[0] ? 4LL * base : x < thr[1] ? 2LL * base : x < thr[2] ? base : base / 2;
+    }
+    __device__ int64_t operator()(int64_t b) const {
+        const int64_t s = b * vbs;
+        const int64_t e = (s + vbs < n) ? s + vbs : n;
+        const int64_t sz = size_of(b);
+        return (e - s + sz - 1) / sz;
+    }
+};
+
+__global__ void k_v6_emit(V6Count f, int64_t nvb, const int64_t* __restrict__ base_idx, int32_t* __restrict__ st,
+                          int32_t* __restrict__ en) {
+    const int64_t b = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (b >= nvb) return;
+    const int64_t s = b * f.vbs;
+    const int64_t e = (s + f.vbs < f.n) ? s + f.vbs : f.n;
+    const int64_t sz = f.size_of(b);
+    int64_t k = base_idx[b];
+    for (int64_t j = s; j < e; j += sz, ++k) {  // PartitioningKernels.cuh:734-741
+        st[k] = static_cast<int32_t>(j);
+        en[k] = static_cast<int32_t>((j + sz < f.n) ? j + sz : f.n);
+    }
+}
+
+__global__ void k_fill_u32(uint32_t* p, int64_t n, uint32_t v) {
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        p[i] = v;
+}
+
+template <class F> int dispatch_dtype(int dtype, F&& f) {
+    switch (dtype) {
+        case GLC_I32: return f(int32_t());
+        case GLC_U32: return f(uint32_t());
+        case GLC_I64: return f(int64_t());
+        case GLC_U64: return f(uint64_t());
+        default: return fail(GLC_ERR_INVALID_ARG, "bad dtype");
+    }
+}
+
+inline unsigned grid_for(int64_t work, int64_t per_block, int64_t cap = 1LL << 30) {
+    int64_t g = (work + per_block - 1) / per_block;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return static_cast<unsigned>(g);
+}
+
+}  // namespace
+
+int launch_fit(int dtype, const void* d_values, glc_column* c, int32_t bs, hipStream_t s) {
+    if (c->P == 0) return GLC_OK;
+    if (bs < 32 || bs > kFitThreads) return fail(GLC_ERR_INVALID_ARG, "fit block size out of range");
+    return dispatch_dtype(dtype, [&](auto t) {
+        using T = decltype(t);
+        const unsigned g = static_cast<unsigned>(c->P < (1LL << 30) ? c->P : (1LL << 30));
+        hipLaunchKernelGGL(k_fit<T>, dim3(g), dim3(kFitThreads), 0, s, static_cast<const T*>(d_values), c->d_start,
+                           c->d_end, c->d_type, c->d_params, c->d_bits, c->d_err, c->P, bs);
+        GLC_HIP(hipGetLastError());
+        return GLC_OK;
+    });
+}
+
+int launch_scan_offsets(glc_column* c, int64_t* d_total, hipStream_t s) {
+    LenBits f{c->d_start, c->d_end, c->d_bits};
+    hipLaunchKernelGGL(k_scan<LenBits>, dim3(1), dim3(kScanThreads), 0, s, f, c->P, c->d_off, d_total);
+    GLC_HIP(hipGetLastError());
+    return GLC_OK;
+}
+
+int launch_build_index(glc_column* c, IndexSummary* d_sum, hipStream_t s) {
+    if (c->P == 0) return GLC_OK;
+    hipLaunchKernelGGL(k_build_index, dim3(grid_for(c->P, 256)), dim3(256), 0, s, c->d_start, c->d_end, c->d_type,
+                       c->d_params, c->d_bits, c->d_off, c->P, c->n, c->d_desc, c->d_tile_part, c->n_index, d_sum);
+    GLC_HIP(hipGetLastError());
+    return GLC_OK;
+}
+
+int launch_pack(int dtype, const void* d_values, glc_column* c, hipStream_t s) {
+    if (c->P == 0 || c->n == 0 || c->total_bits == 0) return GLC_OK;
+    return dispatch_dtype(dtype, [&](auto t) {
+        using T = decltype(t);
+        hipLaunchKernelGGL(k_pack<T>, dim3(grid_for(c->n, kPackTile)), dim3(kPackThreads), 0, s,
+                           static_cast<const T*>(d_values), c->d_desc, c->d_tile_part, c->n, c->d_words);
+        GLC_HIP(hipGetLastError());
+        return GLC_OK;
+    });
+}
+
+int v6_capacity(int64_t n, int32_t base, int64_t* cap, int64_t* nvb) {
+    if (base < 2 || (base & 1)) return fail(GLC_ERR_INVALID_ARG, "V6 base size must be even and >= 2");
+    const int64_t vbs = 8LL * base;
+    *nvb = (n + vbs - 1) / vbs;
+    *cap = *nvb * ((vbs + base / 2 - 1) / (base / 2));  // at most 16 partitions per variance block
+    return GLC_OK;
+}
+
+int launch_v6(int dtype, const void* d_values, int64_t n, int32_t base, int32_t* d_start, int32_t* d_end,
+              int64_t cap, int64_t* d_count, float* d_var, float* d_thr, hipStream_t s) {
+    int64_t need = 0, nvb = 0;
+    int rc = v6_capacity(n, base, &need, &nvb);
+    if (rc) return rc;
+    if (cap < need) return fail(GLC_ERR_INVALID_ARG, "V6 partition buffer too small");
+    const int64_t vbs = 8LL * base;
+    rc = dispatch_dtype(dtype, [&](auto t) {
+        using T = decltype(t);
+        const unsigned g = static_cast<unsigned>(nvb < (1LL << 30) ? nvb : (1LL << 30));
+        hipLaunchKernelGGL(k_v6_var<T>, dim3(g), dim3(kV6Threads), 0, s, static_cast<const T*>(d_values), n, vbs,
+                           nvb, d_var);
+        GLC_HIP(hipGetLastError());
+        return GLC_OK;
+    });
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_v6_select, dim3(1), dim3(1024), 0, s, d_var, nvb, d_thr);
+    GLC_HIP(hipGetLastError());
+    V6Count f{d_var, d_thr, vbs, n, base};
+    // per-block partition base index (exclusive scan) stored after the counts slot
+    int64_t* d_base = d_count + 1;
+    hipLaunchKernelGGL(k_scan<V6Count>, dim3(1), dim3(kScanThreads), 0, s, f, nvb, d_base, d_count);
+    GLC_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_v6_emit, dim3(grid_for(nvb, 256)), dim3(256), 0, s, f, nvb, d_base, d_start, d_end);
+    GLC_HIP(hipGetLastError());
+    return GLC_OK;
+}
+
+int launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s) {
+    if (n <= 0) return GLC_OK;
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, p, n, v);
+    GLC_HIP(hipGetLastError());
+    return GLC_OK;
+}
+
+}  // namespace glc

@@ -1,0 +1,145 @@
+// glc_internal.hpp -- shared definitions of libglc (host + gfx950 device).
+//
+// Device layout of a column (all in HBM, one arena allocation per column):
+//   reference SoA (api/G-LeCo_Types.cuh:18-42):
+//     int32 start[P], end[P], model_type[P]; double params[4P];
+//     int32 delta_bits[P]; int64 bit_offset[P]; int64 error_bound[P]
+//   derived decode index (built once per column):
+//     Desc desc[P]            32 B per partition: theta0, theta1, packed
+//                             (bit_offset | bits<<48 | type<<56), start, len
+//     int32 tile_part[T+1]    T = ceil(N/1024): the last partition whose start
+//                             is <= t*1024 (O(1) partition lookup per tile)
+//   payload: uint32 words[ceil(B/32) + PAD]   (zero padded, 256 B aligned)
+//   optional pre-unpacked deltas: int64 plain[N]
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "glc.h"
+
+namespace glc {
+
+constexpr int kIndexShift = 10;                 // tile_part granularity: 1024 elements
+constexpr int64_t kIndexTile = 1LL << kIndexShift;
+constexpr int kPayloadPadWords = 64;            // >= 2 words read past the last needed word, +256 B slack
+constexpr int kDecodeThreads = 256;             // 4 wave64 per workgroup
+constexpr int kDecodeTile = 2048;               // elements per workgroup tile (8 per thread)
+constexpr int kMaxTileParts = 64;               // partitions of one tile staged in LDS (slow path)
+
+enum { MODEL_CONSTANT = 0, MODEL_LINEAR = 1, MODEL_POLY2 = 2, MODEL_POLY3 = 3, MODEL_DIRECT_COPY = 4 };
+
+struct alignas(32) Desc {
+    double theta0;
+    double theta1;
+    uint64_t packed;   // bit_offset (48 bits) | delta_bits << 48 | model_type << 56
+    int32_t start;
+    int32_t len;
+};
+static_assert(sizeof(Desc) == 32, "Desc is 32 bytes");
+
+__host__ __device__ inline uint64_t desc_pack(int64_t bit_off, int32_t bits, int32_t type) {
+    return (static_cast<uint64_t>(bit_off) & ((1ULL << 48) - 1)) | (static_cast<uint64_t>(bits & 0xFF) << 48) |
+           (static_cast<uint64_t>(type & 0xFF) << 56);
+}
+__host__ __device__ inline int64_t desc_off(uint64_t pk) { return static_cast<int64_t>(pk & ((1ULL << 48) - 1)); }
+__host__ __device__ inline int32_t desc_bits(uint64_t pk) { return static_cast<int32_t>((pk >> 48) & 0xFF); }
+__host__ __device__ inline int32_t desc_type(uint64_t pk) { return static_cast<int32_t>(pk >> 56); }
+
+// Summary computed on the device when the index is built (one D2H copy).
+struct IndexSummary {
+    int32_t max_bits;
+    int32_t malformed;      // unsorted / overlapping / out-of-range partitions
+    int32_t fixed_size;     // S if every partition is [pS, min((p+1)S, N)), else 0
+    int32_t has_poly;       // any POLYNOMIAL2/3 partition (decode-only models)
+    int32_t not_fixed;      // scratch flag for fixed-size detection
+    int32_t bad_bits;       // delta_bits outside [0, 64]
+    int64_t total_bits;     // max end bit over partitions
+};
+
+}  // namespace glc
+
+// Opaque column handle of the C-ABI.
+struct glc_column {
+    int dtype = 0;
+    int elem_size = 4;
+    int device = 0;
+    int64_t n = 0;
+    int64_t P = 0;
+    int64_t total_bits = 0;
+    int64_t payload_words = 0;      // ceil(total_bits / 32)
+    int32_t fixed_size = 0;
+    int32_t max_bits = 0;
+    int32_t fit_block = 0;
+    int32_t has_poly = 0;
+    // arena
+    void* meta_arena = nullptr;
+    int64_t meta_bytes = 0;
+    int32_t* d_start = nullptr;
+    int32_t* d_end = nullptr;
+    int32_t* d_type = nullptr;
+    double* d_params = nullptr;
+    int32_t* d_bits = nullptr;
+    int64_t* d_off = nullptr;
+    int64_t* d_err = nullptr;
+    glc::Desc* d_desc = nullptr;
+    int32_t* d_tile_part = nullptr;
+    int64_t n_index = 0;            // entries of tile_part (ceil(N/1024) + 1)
+    uint32_t* d_words = nullptr;    // payload (nullptr after preunpack with free_packed)
+    int64_t words_alloc = 0;
+    int64_t* d_plain = nullptr;     // pre-unpacked deltas
+};
+
+namespace glc {
+
+// ---- error plumbing ------------------------------------------------------
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+#define GLC_HIP(call)                                                                               \
+    do {                                                                                            \
+        hipError_t e_ = (call);                                                                     \
+        if (e_ != hipSuccess)                                                                       \
+            return ::glc::fail(GLC_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));     \
+    } while (0)
+
+inline int elem_size_of(int dtype) { return (dtype == GLC_I32 || dtype == GLC_U32) ? 4 : 8; }
+inline bool valid_dtype(int dtype) { return dtype >= GLC_I32 && dtype <= GLC_U64; }
+
+// ---- launchers implemented in the .hip translation units -----------------
+// compress side (glc_compress.hip)
+int launch_fit(int dtype, const void* d_values, glc_column* c, int32_t bs, hipStream_t s);
+int launch_scan_offsets(glc_column* c, int64_t* d_total, hipStream_t s);
+int launch_pack(int dtype, const void* d_values, glc_column* c, hipStream_t s);
+int launch_build_index(glc_column* c, IndexSummary* d_sum, hipStream_t s);
+int launch_v6(int dtype, const void* d_values, int64_t n, int32_t base, int32_t* d_start, int32_t* d_end,
+              int64_t cap, int64_t* d_count, float* d_var, float* d_thr, hipStream_t s);
+int v6_capacity(int64_t n, int32_t base, int64_t* cap, int64_t* nvb);
+int launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s);
+// decode side (glc_decode.hip)
+int launch_decode_full(const glc_column* c, void* d_out, int mode, hipStream_t s);
+int launch_preunpack(glc_column* c, int64_t* d_plain, hipStream_t s);
+// access side (glc_access.hip)
+int launch_random_access(const glc_column* c, const void* d_pos, int pos64, int64_t q, void* d_out, int mode,
+                         int32_t fixed_size, hipStream_t s);
+struct BlobView {            // device pointers into a serialized blob
+    const int32_t* start;
+    const int32_t* end;
+    const int32_t* type;
+    const double* params;
+    const int32_t* bits;
+    const int64_t* off;
+    const uint32_t* words;
+    int64_t nwords;
+    int64_t P;
+    int64_t n;
+};
+int launch_direct_access(int dtype, const BlobView& v, const int32_t* d_pos, int64_t q, void* d_out, hipStream_t s);
+
+// host-side CPU partitioner (glc_host.cpp)
+int cpu_split_merge(int dtype, const void* h_values, int64_t n, int32_t min_size, std::vector<int32_t>& st,
+                    std::vector<int32_t>& en);
+
+}  // namespace glc

@@ -1,0 +1,340 @@
+"""Python plumbing for libglc (the MI355X-native G-LeCo library).
+
+This module only moves pointers: device buffers and streams come from torch,
+every computation runs in libglc.so's HIP kernels through the C-ABI of
+include/glc.h.  There is no fallback: if libglc.so is missing, or a call
+fails, an exception is raised.
+
+The method names mirror LeCoGPU<T> (api/G-LeCo.cuh of the reference):
+compress, decompress (random access), decompressFullFile*, randomAccess*,
+serialize/serializeGPU, deserialize/deserializeGPU, createDirectAccessHandle,
+directRandomAccessGPU/CPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.normpath(os.path.join(_HERE, "..", "lib", "libglc.so"))
+
+I32, U32, I64, U64 = 0, 1, 2, 3
+PART_FIXED, PART_V6, PART_CPU_SPLITMERGE, PART_GPU_SPLITMERGE, PART_EXPLICIT = 0, 1, 2, 3, 4
+FULL_AUTO, FULL_BITPACKED, FULL_PREUNPACKED, FULL_V2, FULL_WORKSTEAL, FULL_WORKSTEAL_ADV, FULL_COOP_EXACT = range(7)
+RA_GENERIC, RA_PREUNPACKED = 0, 1
+ERR_NAMES = {0: "OK", -1: "INVALID_ARG", -2: "HIP", -3: "OOM", -4: "FORMAT", -5: "UNSUPPORTED", -6: "BUDGET",
+             -7: "STATE"}
+NP_OF = {I32: np.int32, U32: np.uint32, I64: np.int64, U64: np.uint64}
+ELEM = {I32: 4, U32: 4, I64: 8, U64: 8}
+
+
+class GlcError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"libglc error {code} ({ERR_NAMES.get(code, '?')}): {msg}")
+        self.code = code
+
+
+class ColumnInfo(C.Structure):
+    _fields_ = [("dtype", C.c_int32), ("elem_size", C.c_int32), ("total_values", C.c_int64),
+                ("num_partitions", C.c_int64), ("total_bits", C.c_int64), ("payload_words", C.c_int64),
+                ("compressed_size", C.c_int64), ("fixed_partition_size", C.c_int32), ("has_packed", C.c_int32),
+                ("has_preunpacked", C.c_int32), ("fit_block_size", C.c_int32), ("device_bytes", C.c_int64)]
+
+
+class Opts(C.Structure):
+    _fields_ = [("partitioner", C.c_int32), ("partition_size", C.c_int32), ("v6_base_size", C.c_int32),
+                ("reserved", C.c_int32 * 5)]
+
+
+_lib = None
+
+# (name, restype, argtypes) for every entry point declared in include/glc.h
+_vp, _i64, _i32, _int, _sz = C.c_void_p, C.c_int64, C.c_int32, C.c_int, C.c_size_t
+SIGNATURES = [
+    ("glc_version", C.c_char_p, []),
+    ("glc_last_error", C.c_char_p, []),
+    ("glc_error_string", C.c_char_p, [_int]),
+    ("glc_compress", _int, [_int, _vp, _i64, _vp, _vp, _vp]),
+    ("glc_compress_partitions", _int, [_int, _vp, _i64, _vp, _vp, _i64, _vp, _vp]),
+    ("glc_column_free", _int, [_vp]),
+    ("glc_column_info_get", _int, [_vp, _vp]),
+    ("glc_column_copy_meta", _int, [_vp] * 8),
+    ("glc_column_copy_payload", _int, [_vp, _vp, _i64]),
+    ("glc_partition", _int, [_int, _vp, _i64, _vp, _vp, _vp, _vp, _i64, _vp]),
+    ("glc_decompress_full", _int, [_vp, _vp, _int, _vp]),
+    ("glc_decompress_full_fixed", _int, [_vp, _vp, _i32, _vp]),
+    ("glc_random_access", _int, [_vp, _vp, _i64, _vp, _int, _vp]),
+    ("glc_random_access_i64", _int, [_vp, _vp, _i64, _vp, _int, _vp]),
+    ("glc_random_access_fixed", _int, [_vp, _vp, _i64, _vp, _i32, _int, _vp]),
+    ("glc_preunpack", _int, [_vp, _int, _vp]),
+    ("glc_serialize", _int, [_vp, _vp, _vp]),
+    ("glc_serialize_gpu", _int, [_vp, _vp, _vp, _vp]),
+    ("glc_free_blob", None, [_vp]),
+    ("glc_deserialize", _int, [_vp, _sz, _int, _int, _vp, _vp]),
+    ("glc_header_checksum", C.c_uint32, [_vp, _sz]),
+    ("glc_save_to_file", _int, [_vp, _sz, C.c_char_p]),
+    ("glc_load_from_file", _int, [C.c_char_p, _vp, _vp]),
+    ("glc_direct_handle_create", _int, [_vp, _sz, _int, _int, _vp]),
+    ("glc_direct_random_access_gpu", _int, [_vp, _vp, _i64, _vp, _vp]),
+    ("glc_direct_random_access_cpu", _int, [_vp, _vp, _i64, _vp]),
+    ("glc_direct_handle_destroy", _int, [_vp]),
+    ("glc_batch_compress", _int, [_int, _vp, _vp, _vp, _i32, _vp, _vp]),
+    ("glc_batch_decompress", _int, [_vp, _vp, _vp]),
+    ("glc_batch_decompress_graph", _int, [_vp, _vp, _vp]),
+    ("glc_batch_set_schedule", _int, [_vp, _i32, _i32]),
+    ("glc_batch_get_column", _int, [_vp, _int, _vp]),
+    ("glc_batch_num_columns", _int, [_vp]),
+    ("glc_batch_free", _int, [_vp]),
+]
+
+
+def lib() -> C.CDLL:
+    """Load libglc.so; raises if it has not been built (no fallback exists)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"libglc.so not found at {LIB_PATH}; run __graft_entry__.build() "
+                               "(make -C g-leco_amd/csrc)")
+        L = C.CDLL(LIB_PATH)
+        for name, res, args in SIGNATURES:
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        raise GlcError(rc, lib().glc_last_error().decode(errors="replace"))
+
+
+def _stream(stream=None):
+    """A hipStream_t: None -> torch's current stream, int -> that raw handle (0 = null stream)."""
+    if isinstance(stream, int):
+        return C.c_void_p(stream)
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return C.c_void_p(s.cuda_stream)
+
+
+def _torch_dtype(dt: int):
+    import torch
+    return {I32: torch.int32, U32: torch.uint32, I64: torch.int64, U64: torch.uint64}[dt]
+
+
+def dtype_of(t) -> int:
+    import torch
+    m = {torch.int32: I32, torch.uint32: U32, torch.int64: I64, torch.uint64: U64}
+    if t.dtype not in m:
+        raise TypeError(f"unsupported dtype {t.dtype}; G-LeCo handles int32/uint32/int64/uint64")
+    return m[t.dtype]
+
+
+def _devptr(t):
+    if not t.is_cuda:
+        raise ValueError("expected a device tensor")
+    if not t.is_contiguous():
+        raise ValueError("expected a contiguous tensor")
+    return C.c_void_p(t.data_ptr())
+
+
+def header_checksum(b: bytes) -> int:
+    a = np.frombuffer(b, np.uint8)
+    return lib().glc_header_checksum(a.ctypes.data_as(C.c_void_p), a.size)
+
+
+class Column:
+    """A compressed column resident in HBM (the reference's CompressedData<T>)."""
+
+    def __init__(self, handle: C.c_void_p, dtype: int):
+        self._h = handle
+        self.dtype = dtype
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if self._h:
+            lib().glc_column_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self) -> ColumnInfo:
+        inf = ColumnInfo()
+        _check(lib().glc_column_info_get(self._h, C.byref(inf)))
+        return inf
+
+    @property
+    def n(self) -> int:
+        return self.info().total_values
+
+    @property
+    def num_partitions(self) -> int:
+        return self.info().num_partitions
+
+    def meta(self) -> dict:
+        P = self.num_partitions
+        out = dict(start=np.zeros(P, np.int32), end=np.zeros(P, np.int32), model_type=np.zeros(P, np.int32),
+                   params=np.zeros((P, 4), np.float64), bits=np.zeros(P, np.int32), bit_offset=np.zeros(P, np.int64),
+                   error_bound=np.zeros(P, np.int64))
+        p = lambda a: a.ctypes.data_as(C.c_void_p)
+        _check(lib().glc_column_copy_meta(self._h, p(out["start"]), p(out["end"]), p(out["model_type"]),
+                                          p(out["params"]), p(out["bits"]), p(out["bit_offset"]),
+                                          p(out["error_bound"])))
+        return out
+
+    def payload(self) -> np.ndarray:
+        nw = self.info().payload_words
+        w = np.zeros(nw, np.uint32)
+        _check(lib().glc_column_copy_payload(self._h, w.ctypes.data_as(C.c_void_p), nw))
+        return w
+
+    def decompress_full(self, mode: int = FULL_AUTO, out=None, stream=None):
+        import torch
+        n = self.n
+        if out is None:
+            out = torch.empty(n, dtype=_torch_dtype(self.dtype), device="cuda")
+        _check(lib().glc_decompress_full(self._h, _devptr(out) if n else None, mode, _stream(stream)))
+        return out
+
+    def decompress_full_fixed(self, partition_size: int, out=None, stream=None):
+        import torch
+        n = self.n
+        if out is None:
+            out = torch.empty(n, dtype=_torch_dtype(self.dtype), device="cuda")
+        _check(lib().glc_decompress_full_fixed(self._h, _devptr(out) if n else None, partition_size,
+                                               _stream(stream)))
+        return out
+
+    def random_access(self, pos, mode: int = RA_GENERIC, out=None, stream=None):
+        import torch
+        if out is None:
+            out = torch.empty(pos.numel(), dtype=_torch_dtype(self.dtype), device="cuda")
+        q = pos.numel()
+        fn = lib().glc_random_access_i64 if pos.dtype == torch.int64 else lib().glc_random_access
+        if pos.dtype not in (torch.int32, torch.int64):
+            raise TypeError("positions must be int32 or int64")
+        _check(fn(self._h, _devptr(pos) if q else None, q, _devptr(out) if q else None, mode, _stream(stream)))
+        return out
+
+    def random_access_fixed(self, pos, partition_size: int, preunpacked: bool = False, out=None, stream=None):
+        import torch
+        if out is None:
+            out = torch.empty(pos.numel(), dtype=_torch_dtype(self.dtype), device="cuda")
+        q = pos.numel()
+        _check(lib().glc_random_access_fixed(self._h, _devptr(pos) if q else None, q, _devptr(out) if q else None,
+                                             partition_size, int(preunpacked), _stream(stream)))
+        return out
+
+    def preunpack(self, free_packed: bool = False, stream=None) -> None:
+        _check(lib().glc_preunpack(self._h, int(free_packed), _stream(stream)))
+
+    def serialize(self, gpu: bool = False, stream=None) -> bytes:
+        blob = C.c_void_p()
+        size = C.c_size_t()
+        if gpu:
+            _check(lib().glc_serialize_gpu(self._h, _stream(stream), C.byref(blob), C.byref(size)))
+        else:
+            _check(lib().glc_serialize(self._h, C.byref(blob), C.byref(size)))
+        try:
+            return C.string_at(blob, size.value)
+        finally:
+            lib().glc_free_blob(blob)
+
+
+def _opts(partitioner: int, partition_size: int, base_size: int) -> Opts:
+    o = Opts()
+    o.partitioner = partitioner
+    o.partition_size = partition_size
+    o.v6_base_size = base_size
+    return o
+
+
+def compress(values, partitioner: int = PART_FIXED, partition_size: int = 0, base_size: int = 0,
+             stream=None) -> Column:
+    """LeCoGPU<T>::compress on a device tensor (api/G-LeCo.cuh:349-614)."""
+    dt = dtype_of(values)
+    h = C.c_void_p()
+    o = _opts(partitioner, partition_size, base_size)
+    n = values.numel()
+    _check(lib().glc_compress(dt, _devptr(values) if n else None, n, C.byref(o), _stream(stream), C.byref(h)))
+    return Column(h, dt)
+
+
+def compress_partitions(values, start: np.ndarray, end: np.ndarray, stream=None) -> Column:
+    dt = dtype_of(values)
+    st = np.ascontiguousarray(start, np.int32)
+    en = np.ascontiguousarray(end, np.int32)
+    h = C.c_void_p()
+    _check(lib().glc_compress_partitions(dt, _devptr(values), values.numel(), st.ctypes.data_as(C.c_void_p),
+                                         en.ctypes.data_as(C.c_void_p), st.size, _stream(stream), C.byref(h)))
+    return Column(h, dt)
+
+
+def partition(values, partitioner: int, partition_size: int = 0, base_size: int = 0, stream=None):
+    dt = dtype_of(values)
+    o = _opts(partitioner, partition_size, base_size)
+    P = C.c_int64()
+    _check(lib().glc_partition(dt, _devptr(values), values.numel(), C.byref(o), _stream(stream), None, None, 0,
+                               C.byref(P)))
+    st = np.zeros(P.value, np.int32)
+    en = np.zeros(P.value, np.int32)
+    _check(lib().glc_partition(dt, _devptr(values), values.numel(), C.byref(o), _stream(stream),
+                               st.ctypes.data_as(C.c_void_p), en.ctypes.data_as(C.c_void_p), P.value, C.byref(P)))
+    return st, en
+
+
+def deserialize(blob: bytes, dtype: int, preunpack: bool = False, stream=None) -> Column:
+    a = np.frombuffer(blob, np.uint8)
+    h = C.c_void_p()
+    _check(lib().glc_deserialize(a.ctypes.data_as(C.c_void_p), a.size, dtype, int(preunpack), _stream(stream),
+                                 C.byref(h)))
+    return Column(h, dtype)
+
+
+class DirectHandle:
+    """DirectAccessHandle<T> (api/G-LeCo_Types.cuh:75-107) over a host blob."""
+
+    def __init__(self, blob: bytes, dtype: int, copy_to_device: bool = True):
+        self._buf = np.frombuffer(blob, np.uint8).copy()  # the handle does not own the blob; keep it alive
+        self.dtype = dtype
+        self._h = C.c_void_p()
+        _check(lib().glc_direct_handle_create(self._buf.ctypes.data_as(C.c_void_p), self._buf.size, dtype,
+                                              int(copy_to_device), C.byref(self._h)))
+
+    def random_access_gpu(self, pos, out=None, stream=None):
+        import torch
+        if out is None:
+            out = torch.empty(pos.numel(), dtype=_torch_dtype(self.dtype), device="cuda")
+        q = pos.numel()
+        _check(lib().glc_direct_random_access_gpu(self._h, _devptr(pos) if q else None, q,
+                                                  _devptr(out) if q else None, _stream(stream)))
+        return out
+
+    def random_access_cpu(self, pos: np.ndarray) -> np.ndarray:
+        pos = np.ascontiguousarray(pos, np.int32)
+        out = np.zeros(pos.size, NP_OF[self.dtype])
+        _check(lib().glc_direct_random_access_cpu(self._h, pos.ctypes.data_as(C.c_void_p), pos.size,
+                                                  out.ctypes.data_as(C.c_void_p)))
+        return out
+
+    def close(self):
+        if self._h:
+            lib().glc_direct_handle_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,191 @@
+/*
+ * glc.h -- C-ABI of the MI355X-native G-LeCo compress / decompress library
+ * (libglc.so, hand-written gfx950 HIP kernels).
+ *
+ * Plain pointers and sizes only.  Device buffers are HIP device pointers;
+ * `stream` is a hipStream_t passed as void* (NULL = the null stream).  Every
+ * call returns GLC_OK (0) or a negative GLC_ERR_* code; glc_last_error()
+ * returns a message for the calling thread.  Nothing calls exit().
+ *
+ * Each entry point names the reference interface it replaces (paths relative
+ * to the reference repository, "G" = api/G-LeCo.cuh, class LeCoGPU<T>).  The
+ * reference is a header-only C++ class over host std::vectors; this ABI is
+ * its device-pointer form, and g-leco_amd/cpp/LeCoGPU.hpp layers the
+ * reference's exact method names and host-vector signatures on top of it.
+ *
+ * Data format: identical to the reference's CompressedData<T> SoA
+ * (api/G-LeCo_Types.cuh:18-42) and serialized blob v5 (G:826-1016).
+ */
+#ifndef GLC_H
+#define GLC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes ------------------------------------------------------- */
+#define GLC_OK 0
+#define GLC_ERR_INVALID_ARG -1 /* null pointer, bad dtype/mode, size out of range */
+#define GLC_ERR_HIP -2         /* a HIP runtime call failed */
+#define GLC_ERR_OOM -3         /* device or host allocation failed */
+#define GLC_ERR_FORMAT -4      /* blob: bad magic/version/type size/checksum/bounds; malformed partitions */
+#define GLC_ERR_UNSUPPORTED -5 /* mode needs data the column lacks (e.g. no pre-unpacked deltas) */
+#define GLC_ERR_BUDGET -6      /* serving: request refused by the HBM budget */
+#define GLC_ERR_STATE -7       /* object used in a state that does not allow the call */
+
+/* ---- enums -------------------------------------------------------------- */
+typedef enum { GLC_I32 = 0, GLC_U32 = 1, GLC_I64 = 2, GLC_U64 = 3 } glc_dtype;
+
+/* ModelType, api/G-LeCo_Types.cuh:8-14 */
+enum { GLC_MODEL_CONSTANT = 0, GLC_MODEL_LINEAR = 1, GLC_MODEL_POLYNOMIAL2 = 2, GLC_MODEL_POLYNOMIAL3 = 3,
+       GLC_MODEL_DIRECT_COPY = 4 };
+
+typedef enum {
+    GLC_PART_FIXED = 0,           /* [iS, min((i+1)S, N)), G:401-416 (S is a parameter; the reference hard-codes 4096) */
+    GLC_PART_V6 = 1,              /* GPUVariableLengthPartitionerV6, compression/GpuPartitioner.cuh:34-214 */
+    GLC_PART_CPU_SPLITMERGE = 2,  /* VariableLengthPartitioner on the host, compression/CpuPartitioner.cuh:12-423 */
+    GLC_PART_GPU_SPLITMERGE = 3,  /* same cost model, split candidates evaluated on the GPU */
+    GLC_PART_EXPLICIT = 4         /* caller-supplied partition list (glc_compress_partitions) */
+} glc_partitioner;
+
+typedef struct {
+    int32_t partitioner;        /* glc_partitioner */
+    int32_t partition_size;     /* FIXED: S (default 4096); SPLITMERGE: min partition size (default 128) */
+    int32_t v6_base_size;       /* V6: base size (default 2048, G:386) */
+    int32_t reserved[5];
+} glc_opts;
+
+typedef enum {
+    GLC_FULL_AUTO = 0,          /* decompressFullFile, G:680-693: pre-unpacked if present, else packed (fastest kernel) */
+    GLC_FULL_BITPACKED = 1,     /* decompressFullFile_BitPacked, G:1739-1787 */
+    GLC_FULL_PREUNPACKED = 2,   /* decompressFullFile_PreUnpacked, G:1790-1839 */
+    GLC_FULL_V2 = 3,            /* decompressFullFile_OnTheFly_Optimized_V2, G:1842-1875 (one workgroup per partition) */
+    GLC_FULL_WORKSTEAL = 4,     /* decompressFullFile_WorkStealing, G:166-217 */
+    GLC_FULL_WORKSTEAL_ADV = 5, /* decompressFullFile_WorkStealingAdvanced, G:220-268 */
+    GLC_FULL_COOP_EXACT = 6     /* decompressFullFileCooperative, G:271-345, with the FP64 model (exact) */
+} glc_full_mode;
+
+typedef enum {
+    GLC_RA_GENERIC = 0,         /* decompress(cd, positions, out), G:615-678 */
+    GLC_RA_PREUNPACKED = 1      /* randomAccessPreUnpacked, G:696-766 */
+} glc_ra_mode;
+
+typedef struct glc_column glc_column;             /* CompressedData<T> on the device (+ derived indexes) */
+typedef struct glc_direct_handle glc_direct_handle; /* DirectAccessHandle<T>, api/G-LeCo_Types.cuh:75-107 */
+
+typedef struct {
+    int32_t dtype;              /* glc_dtype */
+    int32_t elem_size;          /* sizeof(T) */
+    int64_t total_values;       /* CompressedData::total_values */
+    int64_t num_partitions;     /* CompressedData::num_partitions */
+    int64_t total_bits;         /* sum of len*bits */
+    int64_t payload_words;      /* ceil(total_bits/32) */
+    int64_t compressed_size;    /* *compressed_size_bytes_result of compress(), G:589-603 */
+    int32_t fixed_partition_size; /* S when partitions are [iS, min((i+1)S,N)), else 0 */
+    int32_t has_packed;         /* bit-packed payload resident */
+    int32_t has_preunpacked;    /* d_plain_deltas resident */
+    int32_t fit_block_size;     /* reduction width used by the fit (G:519-520) */
+    int64_t device_bytes;       /* device memory held by the column */
+} glc_column_info;
+
+/* ---- library ----------------------------------------------------------- */
+const char* glc_version(void);
+const char* glc_last_error(void);
+const char* glc_error_string(int code);
+
+/* ---- compress ------------------------------------------------------------
+ * G:349-614 compress(host_vec, use_variable_partitioning, &size, use_gpu_partitioning).
+ * d_values: device array of n values of `dtype`.  Blocks until the column is
+ * built (the reference syncs too, G:549-553).  opts NULL = fixed, S = 4096.  */
+int glc_compress(int dtype, const void* d_values, int64_t n, const glc_opts* opts, void* stream,
+                 glc_column** out);
+/* Caller-supplied partitions [start[p], end[p]) (host arrays), sorted and
+ * non-overlapping: the reference's compress() after its partition step. */
+int glc_compress_partitions(int dtype, const void* d_values, int64_t n, const int32_t* h_start,
+                            const int32_t* h_end, int64_t num_partitions, void* stream, glc_column** out);
+/* G:793-821 cleanup(). NULL is a no-op. */
+int glc_column_free(glc_column* col);
+int glc_column_info_get(const glc_column* col, glc_column_info* info);
+/* Copy the SoA metadata to host arrays (any may be NULL). params is [4*P]. */
+int glc_column_copy_meta(const glc_column* col, int32_t* start, int32_t* end, int32_t* model_type, double* params,
+                         int32_t* delta_bits, int64_t* bit_offsets, int64_t* error_bounds);
+/* Copy the bit-packed payload (ceil(total_bits/32) words) to the host. */
+int glc_column_copy_payload(const glc_column* col, uint32_t* h_words, int64_t n_words);
+/* Partition boundaries the chosen partitioner would produce, without fitting
+ * (host arrays of capacity `cap`; *num_partitions receives P even if cap is small). */
+int glc_partition(int dtype, const void* d_values, int64_t n, const glc_opts* opts, void* stream, int32_t* h_start,
+                  int32_t* h_end, int64_t cap, int64_t* num_partitions);
+
+/* ---- full decompression ---------------------------------------------------
+ * d_out: device array of total_values values.  Asynchronous on `stream`.   */
+int glc_decompress_full(const glc_column* col, void* d_out, int mode, void* stream);
+/* decompressFullFile_Fix(cd, out, partition_size), G:768-788.  The column
+ * must have been partitioned with exactly that fixed size (validated). */
+int glc_decompress_full_fixed(const glc_column* col, void* d_out, int32_t partition_size, void* stream);
+
+/* ---- random access --------------------------------------------------------
+ * Positions outside [0, total_values) decode to 0 (RandomAccessKernels.cuh:100-103).
+ * All q queries are answered (no grid cap). */
+int glc_random_access(const glc_column* col, const int32_t* d_pos, int64_t q, void* d_out, int mode, void* stream);
+int glc_random_access_i64(const glc_column* col, const int64_t* d_pos, int64_t q, void* d_out, int mode,
+                          void* stream);
+/* randomAccessFixed (G:1878-1928) / randomAccessFixedPreUnpacked (G:1931-1996). */
+int glc_random_access_fixed(const glc_column* col, const int32_t* d_pos, int64_t q, void* d_out,
+                            int32_t partition_size, int preunpacked, void* stream);
+
+/* ---- pre-unpack -------------------------------------------------------------
+ * deserializeGPU(..., preUnpackDeltas=true) stage, G:2249-2288: payload ->
+ * int64 per element.  free_packed != 0 drops the packed payload afterwards,
+ * as the reference does (G:2284-2285). */
+int glc_preunpack(glc_column* col, int free_packed, void* stream);
+
+/* ---- serialization (blob v5) ------------------------------------------------
+ * serialize / serializeGPU (G:826-1016, G:1999-2152): byte-identical output.
+ * *blob is malloc'd by the library; release with glc_free_blob. */
+int glc_serialize(const glc_column* col, uint8_t** blob, size_t* size);
+int glc_serialize_gpu(const glc_column* col, void* stream, uint8_t** blob, size_t* size);
+void glc_free_blob(uint8_t* blob);
+/* deserialize / deserializeGPU (G:1019-1154, G:2157-2302): validates magic,
+ * version (4 or 5), type size, header checksum and every offset against size. */
+int glc_deserialize(const uint8_t* blob, size_t size, int dtype, int preunpack, void* stream, glc_column** out);
+/* io/FileUtils.cuh:5-17 (rotate-add variant, which the reference build uses). */
+uint32_t glc_header_checksum(const void* data, size_t size);
+/* saveToFile / loadFromFile, G:1683-1733. */
+int glc_save_to_file(const uint8_t* data, size_t size, const char* path);
+int glc_load_from_file(const char* path, uint8_t** data, size_t* size);
+
+/* ---- direct access handle (G:1157-1681) ------------------------------------ */
+int glc_direct_handle_create(const uint8_t* blob, size_t size, int dtype, int copy_to_device,
+                             glc_direct_handle** out);
+int glc_direct_random_access_gpu(const glc_direct_handle* h, const int32_t* d_pos, int64_t q, void* d_out,
+                                 void* stream);
+/* directRandomAccessCPU, G:1337-1672: host positions -> host values. */
+int glc_direct_random_access_cpu(const glc_direct_handle* h, const int32_t* h_pos, int64_t q, void* h_out);
+int glc_direct_handle_destroy(glc_direct_handle* h);
+
+/* ---- batched form (no reference counterpart: many columns, one launch) ---- */
+typedef struct glc_batch glc_batch;
+/* Compress `num_columns` device columns (values[i], n[i], dtypes[i]) with fixed
+ * partitions of `partition_size` in a fixed number of launches independent of
+ * the column count. */
+int glc_batch_compress(int num_columns, const int32_t* dtypes, const void* const* d_values, const int64_t* n,
+                       int32_t partition_size, void* stream, glc_batch** out);
+/* Decompress every column of the batch into d_out[i] in one launch. */
+int glc_batch_decompress(const glc_batch* b, void* const* d_out, void* stream);
+/* Capture the batch decompress into a hipGraph and replay it (graph is cached
+ * per output-pointer set). */
+int glc_batch_decompress_graph(glc_batch* b, void* const* d_out, void* stream);
+/* Tile/schedule used by the batched kernels: elements per wave tile (256..8192,
+ * multiple of 64) and waves per workgroup (1..16).  0 = autotuned/default. */
+int glc_batch_set_schedule(glc_batch* b, int32_t tile_elems, int32_t waves_per_group);
+int glc_batch_get_column(const glc_batch* b, int index, const glc_column** col);
+int glc_batch_num_columns(const glc_batch* b);
+int glc_batch_free(glc_batch* b);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GLC_H */

@@ -1,0 +1,102 @@
+"""CPU-side checks of the C-ABI library (no GPU compute).
+
+libglc.so loads, exports every entry point include/glc.h declares, and its
+host-only entry points (header checksum, blob validation, the host direct
+access path of directRandomAccessCPU, api/G-LeCo.cuh:1337-1672) agree with
+the oracle.
+"""
+import ctypes as C
+import os
+import re
+import struct
+
+import numpy as np
+import pytest
+
+import glc
+import oracle as O
+from glc import workloads as W
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    src = open(os.path.join(ROOT, "include", "glc.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(glc_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_library_loads_and_exports_every_declared_symbol():
+    L = glc.lib()
+    names = declared_functions()
+    assert len(names) >= 30
+    for name in names:
+        assert hasattr(L, name), name
+    assert set(names) == {s[0] for s in glc.SIGNATURES}
+    assert b"gfx950" in L.glc_version()
+
+
+def test_header_checksum_matches_oracle_and_reference():
+    rng = np.random.default_rng(3)
+    for n in [0, 1, 120, 4096]:
+        b = rng.integers(0, 256, n, dtype=np.uint8).tobytes()
+        assert glc.header_checksum(b) == O.checksum_rotadd(b)
+        ref = O.ref_checksum(b)
+        if ref is not None:
+            assert glc.header_checksum(b) == ref
+
+
+def _corrupt(blob: bytes, off: int, val: bytes) -> bytes:
+    return blob[:off] + val + blob[off + len(val):]
+
+
+def _reseal(blob: bytes) -> bytes:
+    z = blob[:100] + b"\0\0\0\0" + blob[104:120]
+    return blob[:100] + struct.pack("<I", O.checksum_rotadd(z)) + blob[104:]
+
+
+def test_deserialize_rejects_bad_blobs_before_touching_the_device():
+    v = W.c1_int32(5000)
+    col = O.compress(v, O.PART_FIXED, 1024)
+    blob = col.blob
+    bad = {
+        "short": blob[:100],
+        "magic": _reseal(_corrupt(blob, 0, b"XXXX")),
+        "version": _reseal(_corrupt(blob, 4, struct.pack("<I", 6))),
+        "checksum": _corrupt(blob, 100, struct.pack("<I", 1234)),
+        "truncated": blob[:-16],
+        "offset": _reseal(_corrupt(blob, 16, struct.pack("<Q", 1 << 40))),
+    }
+    for name, b in bad.items():
+        with pytest.raises(glc.GlcError) as e:
+            glc.deserialize(b, glc.I32, stream=0)
+        assert e.value.code == -4, name  # GLC_ERR_FORMAT
+    with pytest.raises(glc.GlcError) as e:
+        glc.deserialize(blob, glc.I64, stream=0)  # type size mismatch
+    assert e.value.code == -4
+
+
+@pytest.mark.parametrize("gen,dt", [("c1", glc.I32), ("c2", glc.U64), ("c3", glc.I64)])
+def test_direct_access_cpu_matches_oracle(gen, dt):
+    n = 30000
+    v = {"c1": lambda: W.c1_int32(n), "c2": lambda: W.c2_uint64(n, nseg=8), "c3": lambda: W.c3_int64(n)}[gen]()
+    col = O.compress(v, O.PART_FIXED, 4096)
+    h = glc.DirectHandle(col.blob, dt, copy_to_device=False)
+    rng = np.random.default_rng(0)
+    pos = np.concatenate([rng.integers(0, n, 4000), [0, n - 1, n, -1]]).astype(np.int32)
+    got = h.random_access_cpu(pos)
+    want = O.random_access(col, pos.astype(np.int64))
+    assert np.array_equal(got, want)
+    h.close()
+
+
+def test_direct_handle_rejects_out_of_range_bit_ranges():
+    v = W.c1_int32(5000)
+    col = O.compress(v, O.PART_FIXED, 1024)
+    blob = bytearray(col.blob)
+    hdr = struct.unpack_from("<4I10Q", blob, 0)
+    bits_off = hdr[8]
+    struct.pack_into("<i", blob, bits_off, 64)  # partition 0 now claims 64 bits per element
+    with pytest.raises(glc.GlcError) as e:
+        glc.DirectHandle(bytes(blob), glc.I32, copy_to_device=False)
+    assert e.value.code == -4

@@ -1,0 +1,233 @@
+"""GPU parity: libglc.so (gfx950 HIP kernels, called through the C-ABI) against
+the oracle, bit-exact: partition boundaries, model parameters (FP64 bits),
+bit widths, bit offsets, error bounds, packed payload, serialized blob bytes,
+and every decoder / random-access variant against the input.
+"""
+import struct
+
+import numpy as np
+import pytest
+import torch
+
+import glc
+import oracle as O
+from glc import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+DT_OF = {np.dtype(np.int32): glc.I32, np.dtype(np.uint32): glc.U32, np.dtype(np.int64): glc.I64,
+         np.dtype(np.uint64): glc.U64}
+PART = {"fixed": (glc.PART_FIXED, O.PART_FIXED), "v6": (glc.PART_V6, O.PART_V6),
+        "cpu": (glc.PART_CPU_SPLITMERGE, O.PART_CPU)}
+
+
+def to_dev(v: np.ndarray):
+    return torch.from_numpy(np.ascontiguousarray(v)).cuda()
+
+
+def host(t) -> np.ndarray:
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def assert_same_column(col, ref, check_fit_block=True):
+    m = col.meta()
+    P = ref.num_partitions
+    assert col.num_partitions == P
+    for k in ("start", "end", "model_type", "bits", "bit_offset", "error_bound"):
+        assert np.array_equal(m[k], getattr(ref, k)), k
+    assert np.array_equal(m["params"].view(np.uint64), ref.params.view(np.uint64)), "theta bits"
+    inf = col.info()
+    assert inf.total_bits == ref.total_bits
+    assert inf.compressed_size == ref.compressed_size()
+    if check_fit_block:
+        assert inf.fit_block_size == ref.fit_block
+    assert np.array_equal(col.payload(), ref.words), "payload"
+
+
+def check_decoders(col, v, ref, fixed_size=None, q=4000, seed=0):
+    n = v.size
+    for mode in (glc.FULL_AUTO, glc.FULL_BITPACKED, glc.FULL_V2, glc.FULL_WORKSTEAL, glc.FULL_WORKSTEAL_ADV):
+        out = host(col.decompress_full(mode))
+        assert np.array_equal(out, v), f"full mode {mode}"
+    rng = np.random.default_rng(seed)
+    pos = np.concatenate([rng.integers(0, max(n, 1), q), [0, n - 1, n, n + 7, -1, -(1 << 31)]]).astype(np.int32)
+    want = O.random_access(ref, pos.astype(np.int64))
+    valid = (pos >= 0) & (pos < n)
+    assert np.array_equal(want[valid], v[pos[valid]]) and np.all(want[~valid] == 0)
+    assert np.array_equal(host(col.random_access(to_dev(pos))), want)
+    assert np.array_equal(host(col.random_access(to_dev(pos.astype(np.int64)))), want)
+    if fixed_size:
+        assert np.array_equal(host(col.decompress_full_fixed(fixed_size)), v)
+        assert np.array_equal(host(col.random_access_fixed(to_dev(pos), fixed_size)), want)
+    # pre-unpacked forms (deserializeGPU(..., true) stage)
+    col.preunpack(free_packed=False)
+    for mode in (glc.FULL_PREUNPACKED, glc.FULL_COOP_EXACT, glc.FULL_AUTO, glc.FULL_V2, glc.FULL_WORKSTEAL):
+        assert np.array_equal(host(col.decompress_full(mode)), v), f"preunpacked mode {mode}"
+    assert np.array_equal(host(col.random_access(to_dev(pos), glc.RA_PREUNPACKED)), want)
+    if fixed_size:
+        assert np.array_equal(host(col.random_access_fixed(to_dev(pos), fixed_size, preunpacked=True)), want)
+    assert np.array_equal(host(col.decompress_full(glc.FULL_BITPACKED)), v)  # packed still resident
+
+
+def full_case(v, part, size=4096, base=2048, fixed_size=None):
+    gp, op = PART[part]
+    col = glc.compress(to_dev(v), gp, size if part != "v6" else 0, base if part == "v6" else 0)
+    ref = O.compress(v, op, size, base)
+    assert_same_column(col, ref)
+    # blob v5: both serializers give the oracle's bytes
+    assert col.serialize() == ref.blob
+    assert col.serialize(gpu=True) == ref.blob
+    check_decoders(col, v, ref, fixed_size=fixed_size)
+    # deserialize the oracle's blob, plain and pre-unpacked
+    d = glc.deserialize(ref.blob, DT_OF[v.dtype])
+    assert_same_column(d, ref, check_fit_block=False)
+    assert np.array_equal(host(d.decompress_full()), v)
+    assert d.serialize() == ref.blob
+    d2 = glc.deserialize(ref.blob, DT_OF[v.dtype], preunpack=True)
+    inf = d2.info()
+    assert np.array_equal(host(d2.decompress_full()), v)
+    if ref.total_bits > 0:  # deserializeGPU pre-unpacks only when a payload exists (G:2249)
+        assert inf.has_preunpacked == 1 and inf.has_packed == 0
+        with pytest.raises(glc.GlcError):
+            d2.serialize()  # G:2006-2011: pre-unpacked objects cannot be re-serialized
+    # direct access handle on the blob (GPU lookup)
+    h = glc.DirectHandle(ref.blob, DT_OF[v.dtype])
+    pos = np.random.default_rng(5).integers(-2, v.size + 2, 3000).astype(np.int32)
+    assert np.array_equal(host(h.random_access_gpu(to_dev(pos))), O.random_access(ref, pos.astype(np.int64)))
+    return col, ref
+
+
+GENS = {
+    "i32": lambda n: W.c1_int32(n),
+    "u32": lambda n: (W.c1_int32(n).astype(np.int64) + (1 << 31)).astype(np.uint32),
+    "i64": lambda n: W.c3_int64(n),
+    "u64": lambda n: W.c2_uint64(n, nseg=32),
+}
+
+
+@pytest.mark.parametrize("gen", sorted(GENS))
+@pytest.mark.parametrize("part,size", [("fixed", 4096), ("fixed", 1024), ("fixed", 1000), ("v6", 0), ("cpu", 128)])
+def test_parity_small(gpu, gen, part, size):
+    n = 50_000
+    v = GENS[gen](n)
+    full_case(v, part, size or 4096, fixed_size=(size if part == "fixed" else None))
+
+
+def test_parity_c1_config(gpu):
+    # int32, N = 2^20, fixed 1024-value partitions, 10^5 uniform lookups
+    v = W.c1_int32(1 << 20)
+    col, ref = full_case(v, "fixed", 1024, fixed_size=1024)
+    pos = W.positions_uniform(100_000, v.size)
+    assert np.array_equal(host(col.random_access(to_dev(pos))), v[pos])
+    assert int(ref.bits.max()) <= 9
+
+
+@pytest.mark.parametrize("n", [1, 2, 31, 33, 70, 100, 255, 257, 1023, 1025, 4097])
+def test_parity_tiny_lengths(gpu, n):
+    # n < 256 exercises the reference's block-size rule (bs = roundup32(len0), truncated tree at 96, 160, ...)
+    rng = np.random.default_rng(n)
+    v = (1000 + 9 * np.arange(n) + rng.integers(-40, 40, n)).astype(np.int64)
+    full_case(v, "fixed", 4096, fixed_size=4096)
+    full_case(v, "fixed", 64, fixed_size=64)
+
+
+def test_parity_wide_and_direct_copy(gpu):
+    rng = np.random.default_rng(1)
+    n = 20_000
+    # uint64 above 2^53 in some partitions -> DIRECT_COPY with 64 bits
+    u = W.c2_uint64(n, nseg=4)
+    u[4096:8192] += np.uint64(1 << 60)
+    col, ref = full_case(u, "fixed", 4096, fixed_size=4096)
+    assert ref.model_type[1] == O.MODEL_DIRECT_COPY and ref.bits[1] == 64
+    # int64 residuals needing 33..64 bits
+    w = (rng.integers(-(1 << 62), 1 << 62, n)).astype(np.int64)
+    w[:4096] = rng.integers(-(1 << 31), 1 << 31, 4096)
+    col, ref = full_case(w, "fixed", 4096, fixed_size=4096)
+    assert ref.bits.max() >= 63
+    # int32 extremes -> 33-bit deltas; bits == 32 exactly
+    e = np.array([2**31 - 1, -2**31] * 2048, np.int32)
+    full_case(e, "fixed", 1024, fixed_size=1024)
+    z = np.zeros(8192, np.int64)
+    z[::2] = (1 << 30) + 5
+    z[1::2] = -(1 << 30) - 5
+    col, ref = full_case(z, "fixed", 4096, fixed_size=4096)
+    assert ref.bits[0] == 32
+    # constant column -> 0 bits, empty payload
+    c = np.full(10_000, 7, np.uint32)
+    col, ref = full_case(c, "fixed", 4096, fixed_size=4096)
+    assert ref.total_bits == 0
+
+
+def test_parity_explicit_partitions_with_gaps(gpu):
+    v = W.c3_int64(30_000)
+    st = np.array([0, 100, 5000, 5000, 9000, 20000], np.int32)
+    en = np.array([100, 4000, 5000, 8000, 15000, 29000], np.int32)  # gaps: [4000,5000), [8000,9000), ...
+    col = glc.compress_partitions(to_dev(v), st, en)
+    ref = O.compress_parts(v, st, en)
+    assert_same_column(col, ref)
+    want = O.decode_full(ref)
+    for mode in (glc.FULL_AUTO, glc.FULL_V2, glc.FULL_WORKSTEAL, glc.FULL_WORKSTEAL_ADV):
+        assert np.array_equal(host(col.decompress_full(mode)), want)
+    pos = np.arange(0, 30_000, 7, dtype=np.int32)
+    assert np.array_equal(host(col.random_access(to_dev(pos))), O.random_access(ref, pos.astype(np.int64)))
+    assert col.serialize() == ref.blob
+    with pytest.raises(glc.GlcError):  # malformed: overlapping partitions
+        glc.compress_partitions(to_dev(v), np.array([0, 50], np.int32), np.array([100, 200], np.int32))
+
+
+def _reseal(b: bytearray) -> bytes:
+    b[100:104] = b"\0\0\0\0"
+    b[100:104] = struct.pack("<I", O.checksum_rotadd(bytes(b[:120])))
+    return bytes(b)
+
+
+def test_parity_decode_only_polynomial_models(gpu):
+    # POLYNOMIAL2/3 are decoded but never produced (FullDecompressionKernels.cuh:164-167);
+    # craft them into a blob and compare every GPU decoder with the oracle's decode rule.
+    v = W.c1_int32(12_000)
+    ref = O.compress(v, O.PART_FIXED, 4096)
+    b = bytearray(ref.blob)
+    hdr = struct.unpack_from("<4I10Q", b, 0)
+    types_off, params_off = hdr[6], hdr[7]
+    struct.pack_into("<i", b, types_off + 4, O.MODEL_POLY2)
+    struct.pack_into("<d", b, params_off + 32 + 16, 0.001)
+    struct.pack_into("<i", b, types_off + 8, O.MODEL_POLY3)
+    struct.pack_into("<d", b, params_off + 64 + 16, 5.0)
+    blob = _reseal(b)
+    ref.model_type[1], ref.params[1, 2] = O.MODEL_POLY2, 0.001
+    ref.model_type[2], ref.params[2, 2] = O.MODEL_POLY3, 5.0
+    want = O.decode_full(ref)
+    assert not np.array_equal(want, v)
+    d = glc.deserialize(blob, glc.I32)
+    for mode in (glc.FULL_AUTO, glc.FULL_V2, glc.FULL_WORKSTEAL, glc.FULL_WORKSTEAL_ADV):
+        assert np.array_equal(host(d.decompress_full(mode)), want)
+    pos = np.arange(0, 12_000, 3, dtype=np.int32)
+    assert np.array_equal(host(d.random_access(to_dev(pos))), want[pos])
+    d.preunpack()
+    assert np.array_equal(host(d.decompress_full(glc.FULL_PREUNPACKED)), want)
+    assert np.array_equal(host(d.random_access(to_dev(pos), glc.RA_PREUNPACKED)), want[pos])
+    h = glc.DirectHandle(blob, glc.I32)
+    assert np.array_equal(host(h.random_access_gpu(to_dev(pos))), want[pos])
+
+
+def test_empty_column_and_errors(gpu):
+    col = glc.compress(to_dev(np.zeros(0, np.int64)))
+    assert col.n == 0 and col.num_partitions == 0
+    out = col.decompress_full()
+    assert out.numel() == 0
+    v = W.c1_int32(5000)
+    col = glc.compress(to_dev(v), glc.PART_FIXED, 1024)
+    with pytest.raises(glc.GlcError) as e:
+        col.decompress_full(glc.FULL_PREUNPACKED)  # no pre-unpacked deltas yet
+    assert e.value.code == -5
+    with pytest.raises(glc.GlcError):
+        col.decompress_full_fixed(4096)  # partition size does not match the column
+    with pytest.raises(glc.GlcError):
+        col.random_access_fixed(to_dev(np.zeros(4, np.int32)), 512)
+    with pytest.raises(glc.GlcError):
+        glc.compress(to_dev(v), glc.PART_GPU_SPLITMERGE)
+    col.preunpack(free_packed=True)
+    assert np.array_equal(host(col.decompress_full()), v)
+    with pytest.raises(glc.GlcError):
+        col.decompress_full(glc.FULL_BITPACKED)
