@@ -1,0 +1,189 @@
+#!/usr/bin/env python3
+"""Benchmark of the G-LeCo hot path on one MI355X.
+
+Workload: uint64 piecewise-linear column, N = 2^28 (2 GiB raw), 4096
+segments (lengths log-uniform in [256, 65536], slopes U[1, 2^20], noise
+U[-2^b, 2^b], b in 0..16), encoded with the V6 variable-length partitioner.
+One step = one work-stealing full decompress of the column into HBM plus one
+batch of 2^24 random lookups (half uniform, half Zipf(1.1)).  Inputs are
+resident in HBM before the timed region; the decode and lookup kernels are
+timed with HIP events on the stream they run on.
+
+value = full-decompress output GB/s (N * 8 bytes / decode time, the
+reference's convention); extra fields give Mlookups/s and p50/p99 lookup
+batch latency.  roofline.achieved = algorithmic bytes of one decode launch
+(payload bytes + 36 B per partition + N * 8 output bytes) / its average
+duration, against 8 TB/s.  cpu_baseline = the CPU restatement (oracle)
+decoding a bounded prefix sample of the same column on the host's cores.
+"""
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "g-leco_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+HBM_PEAK_GBS = 8000.0
+META_BYTES_PER_PARTITION = 36
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(x_dev, n_sample, budget_s=12.0):
+    """Oracle (CPU restatement) decode + lookups on a prefix sample, 1 thread and all cores."""
+    import numpy as np
+    import oracle as O
+    from glc import workloads as W
+
+    v = x_dev[:n_sample].cpu().numpy()
+    t0 = time.perf_counter()
+    col = O.compress(v, O.PART_V6, base_size=2048)
+    comp_s = time.perf_counter() - t0
+    cores = O.max_threads()
+    res = {}
+    for threads in (1, cores):
+        times = []
+        t_end = time.perf_counter() + budget_s / 2
+        while len(times) < 3 or (time.perf_counter() < t_end and len(times) < 10):
+            t = time.perf_counter()
+            out = O.decode_full(col, threads=threads)
+            times.append(time.perf_counter() - t)
+        assert np.array_equal(out, v)
+        res[threads] = n_sample * 8 / statistics.median(times) / 1e9
+    pos = W.positions_mixed(1 << 20, n_sample)
+    t = time.perf_counter()
+    r = O.random_access(col, pos.astype(np.int64), threads=cores)
+    look_s = time.perf_counter() - t
+    assert np.array_equal(r, v[pos])
+    return {"value": round(res[cores], 3), "unit": "GB/s", "cores": cores, "kind": "port",
+            "sample": f"oracle (C++ restatement) full decode of the first {n_sample} values of the same column "
+                      f"(V6-compressed on the host in {comp_s:.1f} s), median of repeated passes",
+            "value_1thread": round(res[1], 3),
+            "lookups_mlookups_per_s": round(len(pos) / look_s / 1e6, 2)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--log2n", type=int, default=28)
+    ap.add_argument("--lookups-log2", type=int, default=24)
+    ap.add_argument("--mode", default="worksteal", choices=["worksteal", "bitpacked", "v2", "worksteal_adv"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample-log2", type=int, default=24)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}). "
+              "Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import glc
+    from glc import workloads as W
+
+    dev = torch.device("cuda:0")
+    n = 1 << args.log2n
+    q = 1 << args.lookups_log2
+    log(f"generating c2 uint64 column, N=2^{args.log2n}")
+    x = W.c2_uint64_torch(n, device=dev)
+    torch.cuda.synchronize()
+    log("compressing (V6, base 2048)")
+    t = time.perf_counter()
+    col = glc.compress(x, glc.PART_V6, 0, 2048)
+    torch.cuda.synchronize()
+    compress_ms = (time.perf_counter() - t) * 1e3
+    inf = col.info()
+    log(f"P={inf.num_partitions} total_bits={inf.total_bits} ratio={n * 8 / inf.compressed_size:.3f}")
+    pos = torch.from_numpy(W.positions_mixed(q, n)).to(dev)
+    out = torch.empty(n, dtype=torch.uint64, device=dev)
+    look = torch.empty(q, dtype=torch.uint64, device=dev)
+    mode = {"worksteal": glc.FULL_WORKSTEAL, "bitpacked": glc.FULL_BITPACKED, "v2": glc.FULL_V2,
+            "worksteal_adv": glc.FULL_WORKSTEAL_ADV}[args.mode]
+
+    for _ in range(args.warmup):
+        col.decompress_full(mode, out=out)
+        col.random_access(pos, out=look)
+    torch.cuda.synchronize()
+    # correctness of what is timed (outside the timed region)
+    assert torch.equal(out.view(torch.int64), x.view(torch.int64)), "decode mismatch"
+    assert torch.equal(look.view(torch.int64), x.view(torch.int64)[pos.long()]), "lookup mismatch"
+    log("verified decode and lookups bit-exact; timing")
+
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True),
+           torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        a, b, c = ev[k]
+        a.record()
+        col.decompress_full(mode, out=out)
+        b.record()
+        col.random_access(pos, out=look)
+        c.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    dec_ms = [a.elapsed_time(b) for a, b, _ in ev]
+    look_ms = [b.elapsed_time(c) for _, b, c in ev]
+    dec_avg = sum(dec_ms) / len(dec_ms)
+    value = n * 8 / (dec_avg * 1e-3) / 1e9
+    alg_bytes = (inf.total_bits + 7) // 8 + META_BYTES_PER_PARTITION * inf.num_partitions + n * 8
+    achieved = alg_bytes / (dec_avg * 1e-3) / 1e9
+    look_sorted = sorted(look_ms)
+    p = lambda f: look_sorted[min(len(look_sorted) - 1, int(round(f * (len(look_sorted) - 1))))]
+    look_avg = sum(look_ms) / len(look_ms)
+
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "decode_traffic.json")
+    if os.path.exists(tf):
+        try:
+            traffic = json.load(open(tf)).get("bytes_per_launch")
+        except Exception:
+            traffic = None
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline (oracle on a bounded sample)")
+        cpu = cpu_baseline(x, min(n, 1 << args.cpu_sample_log2))
+
+    line = {
+        "metric": "full-decompress GB/s (% HBM roofline) + random-access Mlookups/s; p99 lookup ms",
+        "value": round(value, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3 / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic (seeded generator of the configured shape, generated on the GPU)",
+        "config": {"workload": f"c2_uint64_piecewise_linear_N2^{args.log2n}_v6", "n": n, "partitioner": "v6",
+                   "v6_base": 2048, "decoder": args.mode, "lookups_per_step": q,
+                   "lookup_mix": "1/2 uniform + 1/2 zipf(1.1) scrambled", "partitions": inf.num_partitions,
+                   "compressed_bytes": inf.compressed_size, "compression_ratio": round(n * 8 / inf.compressed_size, 4)},
+        "decode_ms_avg": round(dec_avg, 4),
+        "decode_ms_min": round(min(dec_ms), 4),
+        "lookup_mlookups_per_s": round(q / (look_avg * 1e-3) / 1e6, 1),
+        "lookup_batch_ms_p50": round(p(0.5), 4),
+        "lookup_batch_ms_p99": round(p(0.99), 4),
+        "compress_ms": round(compress_ms, 2),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": f"k_decode ({args.mode}, uint64, bit-packed)", "algorithmic_bytes": alg_bytes},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

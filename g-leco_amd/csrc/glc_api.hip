@@ -30,7 +30,7 @@ thread_local std::string g_last_error;
 int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
 
 struct ArenaLayout {
-    int64_t start, end, type, bits, params, off, err, desc, tile, summary, total;
+    int64_t start, end, type, bits, params, off, err, desc, tile, tinfo, sched, summary, total;
 };
 
 ArenaLayout layout_for(int64_t P, int64_t n_index) {
@@ -50,6 +50,8 @@ ArenaLayout layout_for(int64_t P, int64_t n_index) {
     L.err = take(P * 8);
     L.desc = take(P * 32);
     L.tile = take((n_index + 2) * 4);  // + 2 work-stealing counters
+    L.tinfo = take(std::max<int64_t>(n_index - 1, 1) * static_cast<int64_t>(sizeof(TileInfo)));
+    L.sched = take(kSchedBytes);
     L.summary = take(static_cast<int64_t>(sizeof(IndexSummary)) + 64);
     L.total = cur;
     return L;
@@ -76,6 +78,8 @@ int alloc_column(glc_column* c, int64_t P, hipStream_t s) {
     c->d_err = reinterpret_cast<int64_t*>(b + L.err);
     c->d_desc = reinterpret_cast<Desc*>(b + L.desc);
     c->d_tile_part = reinterpret_cast<int32_t*>(b + L.tile);
+    c->d_tinfo = reinterpret_cast<TileInfo*>(b + L.tinfo);
+    c->d_sched = reinterpret_cast<int32_t*>(b + L.sched);
     GLC_HIP(hipMemsetAsync(arena, 0, L.total, s));
     return GLC_OK;
 }
@@ -121,7 +125,7 @@ int finish_index(glc_column* c, hipStream_t s, IndexSummary* out) {
     }
     c->max_bits = out->max_bits;
     c->has_poly = out->has_poly;
-    return GLC_OK;
+    return launch_build_tinfo(c, s);
 }
 
 // api/G-LeCo.cuh:519-520

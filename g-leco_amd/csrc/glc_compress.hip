@@ -207,6 +207,27 @@ __global__ void k_build_index(const int32_t* __restrict__ st, const int32_t* __r
     for (int64_t t = t0; t < t1; ++t) tile_part[t] = static_cast<int32_t>(p);
 }
 
+// TileInfo per 1024-element tile (after the descriptors and tile_part exist).
+__global__ void k_build_tinfo(const Desc* __restrict__ desc, const int32_t* __restrict__ tile_part, int64_t n,
+                              TileInfo* __restrict__ ti, int64_t ntiles) {
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= ntiles) return;
+    const int64_t e0 = t * kIndexTile;
+    const int64_t e1 = (e0 + kIndexTile < n) ? e0 + kIndexTile : n;
+    const int32_t p = tile_part[t];
+    const Desc d = desc[p];
+    const int b = desc_bits(d.packed);
+    const int type = desc_type(d.packed);
+    const bool inside = e0 >= d.start && e1 <= static_cast<int64_t>(d.start) + d.len;
+    int path = TILE_SLOW;
+    if (inside && type != MODEL_POLY2) path = (b <= 32 && type != MODEL_DIRECT_COPY) ? TILE_NARROW : TILE_WIDE;
+    TileInfo x;
+    x.W = inside ? desc_off(d.packed) + (e0 - d.start) * b : 0;
+    x.p = p;
+    x.info = b | (path << 8);
+    ti[t] = x;
+}
+
 // ---------------------------------------------------------------------------
 // Pack.  One workgroup per 2048-element tile.
 // ---------------------------------------------------------------------------
@@ -461,6 +482,15 @@ int launch_build_index(glc_column* c, IndexSummary* d_sum, hipStream_t s) {
     if (c->P == 0) return GLC_OK;
     hipLaunchKernelGGL(k_build_index, dim3(grid_for(c->P, 256)), dim3(256), 0, s, c->d_start, c->d_end, c->d_type,
                        c->d_params, c->d_bits, c->d_off, c->P, c->n, c->d_desc, c->d_tile_part, c->n_index, d_sum);
+    GLC_HIP(hipGetLastError());
+    return GLC_OK;
+}
+
+int launch_build_tinfo(glc_column* c, hipStream_t s) {
+    const int64_t ntiles = c->n_index - 1;
+    if (c->P == 0 || ntiles <= 0) return GLC_OK;
+    hipLaunchKernelGGL(k_build_tinfo, dim3(grid_for(ntiles, 256)), dim3(256), 0, s, c->d_desc, c->d_tile_part, c->n,
+                       c->d_tinfo, ntiles);
     GLC_HIP(hipGetLastError());
     return GLC_OK;
 }

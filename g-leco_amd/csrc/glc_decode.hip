@@ -45,7 +45,10 @@ struct DecArgs {
     int64_t n;
     int64_t P;
     int64_t n_index;
-    int32_t* counter;                    // work-stealing counters [2]
+    int32_t* counter;                    // work-stealing counters [2] (workgroup schedules)
+    const TileInfo* __restrict__ tinfo;  // per-1024-element tile: first bit, bits, partition, path
+    int64_t wlimit;                      // payload words (row-load base clamp)
+    int32_t* sched;                      // per-XCD work-stealing queues (wave schedule)
 };
 
 // Decode elements [e0, e1) of partition `d` (caller guarantees e0 >= d.start
@@ -225,6 +228,259 @@ __global__ void k_fill_gaps(DecArgs a) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Wave-level decoder: no LDS staging and no workgroup barriers.
+// A wave owns a 1024-element tile (16 rows of 64 elements).  The tile's
+// TileInfo (first bit, bit width, partition, path) is one 16-byte load.  For
+// each row the wave issues one fully coalesced dword load of the row's words
+// (lane l loads word l); all 16 row loads are issued before any compute, so
+// the wave never waits on its own stores (vmcnt is in-order on gfx9).  Each
+// lane then picks the 2-3 words its element needs with ds_bpermute, extracts
+// the field with v_alignbit_b32 + v_bfe_i32, and stores with a non-temporal
+// store (64 consecutive elements per row: fully coalesced).
+// ---------------------------------------------------------------------------
+constexpr int kWaveRows = 16;
+constexpr int kWaveTile = 64 * kWaveRows;  // == kIndexTile: a wave tile starts on an index entry
+constexpr int kWavesPerGroup = 4;
+static_assert(kWaveTile == kIndexTile, "wave tiles are aligned to the partition index");
+
+__device__ __forceinline__ uint32_t bperm(uint32_t v, uint32_t src_lane) {
+    return static_cast<uint32_t>(
+        __builtin_amdgcn_ds_bpermute(static_cast<int>((src_lane & 63u) << 2), static_cast<int>(v)));
+}
+
+template <class V> __device__ __forceinline__ void put(V* p, V v) { __builtin_nontemporal_store(v, p); }
+
+template <class T, int OP>
+__device__ __forceinline__ void emit(const DecArgs& a, int64_t e, int32_t type, double t0, double t1, int32_t i,
+                                     uint64_t raw, int64_t delta) {
+    if constexpr (OP == OP_UNPACK) {
+        put(static_cast<int64_t*>(a.out) + e,
+            (type == MODEL_DIRECT_COPY) ? static_cast<int64_t>(static_cast<T>(raw)) : delta);
+    } else {
+        put(static_cast<T*>(a.out) + e, (type == MODEL_DIRECT_COPY) ? static_cast<T>(raw)
+                                                                    : apply_delta<T>(dec_pred<T>(t0, t1, i), delta));
+    }
+}
+
+// Narrow fast path: the tile lies in one partition with bits <= 32 (linear model).
+template <class T, int OP>
+__device__ __forceinline__ void wave_narrow(const DecArgs& a, const TileInfo& x, int64_t e0, int64_t e1, int lane) {
+    const int b = tile_bits(x.info);
+    const uint32_t* __restrict__ w = a.words;
+    uint32_t x0[kWaveRows], xs[kWaveRows];
+    // Branch-free loads: lane l reads word wb + l of the row, plus word wb + 64.
+    // wb is clamped to the payload end (rows past the column end are never
+    // stored), and the payload carries kPayloadPadWords zero words, so every
+    // read stays inside the allocation.
+#pragma unroll
+    for (int r = 0; r < kWaveRows; ++r) {
+        const int64_t R = x.W + static_cast<int64_t>(r * 64) * b;
+        const int64_t wb = (R >> 5) < a.wlimit ? (R >> 5) : a.wlimit;
+        x0[r] = w[wb + lane];
+        xs[r] = w[wb + 64];
+    }
+    const Desc d = a.desc[x.p];
+    const int32_t type = desc_type(d.packed);
+    const int32_t lbase = static_cast<int32_t>(e0 - d.start);
+    const uint32_t rel_lane = static_cast<uint32_t>(lane) * static_cast<uint32_t>(b);
+#pragma unroll
+    for (int r = 0; r < kWaveRows; ++r) {
+        const int64_t e = e0 + r * 64 + lane;
+        const int64_t R = x.W + static_cast<int64_t>(r * 64) * b;
+        const uint32_t rel = static_cast<uint32_t>(R & 31) + rel_lane;
+        const uint32_t wi = rel >> 5;
+        const uint32_t lo = bperm(x0[r], wi);
+        const uint32_t hb = bperm(x0[r], wi + 1);
+        const uint32_t hi = (wi + 1 < 64) ? hb : xs[r];
+        const uint32_t raw = __builtin_amdgcn_alignbit(hi, lo, rel & 31);
+        uint64_t rawv;
+        int64_t delta;
+        if (b == 32) {
+            rawv = raw;
+            delta = static_cast<int32_t>(raw);
+        } else {
+            rawv = raw & ((1u << b) - 1u);
+            // the builtin returns an unsigned: sign-extend through int32
+            delta = static_cast<int32_t>(__builtin_amdgcn_sbfe(static_cast<int>(raw), 0u, static_cast<unsigned>(b)));
+        }
+        if (e < e1) emit<T, OP>(a, e, type, d.theta0, d.theta1, lbase + r * 64 + lane, rawv, delta);
+    }
+}
+
+// Wide fast path: one partition, 33..64-bit fields or DIRECT_COPY; two row
+// registers per row (words l and 64 + l) and three bpermute pairs per element.
+template <class T, int OP>
+__device__ __forceinline__ void wave_wide(const DecArgs& a, const TileInfo& x, int64_t e0, int64_t e1, int lane) {
+    const int b = tile_bits(x.info);
+    const uint32_t* __restrict__ w = a.words;
+    const Desc d = a.desc[x.p];
+    const int32_t type = desc_type(d.packed);
+    const int32_t lbase = static_cast<int32_t>(e0 - d.start);
+    const uint32_t rel_lane = static_cast<uint32_t>(lane) * static_cast<uint32_t>(b);
+    for (int r0 = 0; r0 < kWaveRows; r0 += 4) {
+        uint32_t x0[4], x1[4], xs[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {  // branch-free, clamped as in wave_narrow
+            const int64_t R = x.W + static_cast<int64_t>((r0 + k) * 64) * b;
+            const int64_t wb = (R >> 5) < a.wlimit ? (R >> 5) : a.wlimit;
+            x0[k] = w[wb + lane];
+            x1[k] = w[wb + 64 + lane];
+            xs[k] = w[wb + 128];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int64_t e = e0 + (r0 + k) * 64 + lane;
+            const int64_t R = x.W + static_cast<int64_t>((r0 + k) * 64) * b;
+            const uint32_t rel = static_cast<uint32_t>(R & 31) + rel_lane;
+            const uint32_t wi = rel >> 5;
+            const uint32_t w1 = wi + 1, w2 = wi + 2;
+            const uint32_t l0 = bperm(x0[k], wi), l1 = bperm(x1[k], wi);
+            const uint32_t a0 = bperm(x0[k], w1), a1 = bperm(x1[k], w1);
+            const uint32_t c0 = bperm(x0[k], w2), c1 = bperm(x1[k], w2);
+            const uint32_t lo = (wi < 64) ? l0 : l1;
+            const uint32_t hi = (w1 < 64) ? a0 : a1;
+            const uint32_t top = (w2 < 64) ? c0 : (w2 < 128) ? c1 : xs[k];
+            const uint32_t s = rel & 31;
+            const uint64_t raw = (static_cast<uint64_t>(__builtin_amdgcn_alignbit(top, hi, s)) << 32) |
+                                 __builtin_amdgcn_alignbit(hi, lo, s);
+            const uint64_t rawv = (b >= 64) ? raw : (raw & ((1ULL << b) - 1ULL));
+            const int64_t delta =
+                (b >= 64) ? static_cast<int64_t>(raw) : static_cast<int64_t>(raw << (64 - b)) >> (64 - b);
+            if (e < e1) emit<T, OP>(a, e, type, d.theta0, d.theta1, lbase + (r0 + k) * 64 + lane, rawv, delta);
+        }
+    }
+}
+
+// Pre-unpacked deltas: one coalesced 8-byte load per element.
+template <class T>
+__device__ __forceinline__ void wave_plain(const DecArgs& a, const TileInfo& x, int64_t e0, int64_t e1, int lane) {
+    int64_t pl[kWaveRows];
+#pragma unroll
+    for (int r = 0; r < kWaveRows; ++r) {
+        const int64_t e = e0 + r * 64 + lane;
+        pl[r] = (e < e1) ? a.plain[e] : 0;
+    }
+    const Desc d = a.desc[x.p];
+    const int32_t type = desc_type(d.packed);
+    const int32_t lbase = static_cast<int32_t>(e0 - d.start);
+    T* __restrict__ out = static_cast<T*>(a.out);
+#pragma unroll
+    for (int r = 0; r < kWaveRows; ++r) {
+        const int64_t e = e0 + r * 64 + lane;
+        const T v = (type == MODEL_DIRECT_COPY) ? static_cast<T>(pl[r])
+                                                : apply_delta<T>(dec_pred<T>(d.theta0, d.theta1, lbase + r * 64 + lane), pl[r]);
+        if (e < e1) put(out + e, v);
+    }
+}
+
+// Slow path: a tile spanning several partitions, gaps, or a decode-only
+// polynomial model.  Per-lane partition search inside the tile's candidate
+// range, direct extraction from global memory.
+template <class T, int OP>
+__device__ __forceinline__ void wave_slow(const DecArgs& a, int64_t e0, int64_t e1, int64_t plo, int lane) {
+    const int64_t phi = a.tile_part[(e1 + kIndexTile - 1) >> kIndexShift];
+    for (int64_t e = e0 + lane; e < e1; e += 64) {
+        int64_t lo = plo, hi = phi;
+        while (lo < hi) {
+            const int64_t m = (lo + hi + 1) >> 1;
+            if (a.desc[m].start <= e) lo = m; else hi = m - 1;
+        }
+        const Desc d = a.desc[lo];
+        const int64_t k = e - d.start;
+        if (k < 0 || k >= d.len) {
+            if constexpr (OP == OP_UNPACK) static_cast<int64_t*>(a.out)[e] = 0;
+            else static_cast<T*>(a.out)[e] = T(0);
+            continue;
+        }
+        const int32_t type = desc_type(d.packed);
+        const int b = desc_bits(d.packed);
+        const int32_t i = static_cast<int32_t>(k);
+        if constexpr (OP == OP_PLAIN) {
+            const int64_t pl = a.plain[e];
+            T v;
+            if (type == MODEL_DIRECT_COPY) v = static_cast<T>(pl);
+            else if (type == MODEL_POLY2)
+                v = apply_delta<T>(dec_pred_poly<T>(type, d.theta0, d.theta1, a.params[4 * lo + 2], i), pl);
+            else v = apply_delta<T>(dec_pred<T>(d.theta0, d.theta1, i), pl);
+            static_cast<T*>(a.out)[e] = v;
+        } else {
+            const uint64_t raw = (b > 0) ? extract_raw_g(a.words, desc_off(d.packed) + k * b, b) : 0;
+            const int64_t delta = sign_extend(raw, b);
+            if (OP == OP_PACKED && type == MODEL_POLY2) {
+                static_cast<T*>(a.out)[e] =
+                    apply_delta<T>(dec_pred_poly<T>(type, d.theta0, d.theta1, a.params[4 * lo + 2], i), delta);
+            } else {
+                emit<T, OP>(a, e, type, d.theta0, d.theta1, i, raw, delta);
+            }
+        }
+    }
+}
+
+template <class T, int OP> __device__ __forceinline__ void wave_tile(const DecArgs& a, int64_t t, int lane) {
+    const int64_t e0 = t * kWaveTile;
+    const int64_t e1 = (e0 + kWaveTile < a.n) ? e0 + kWaveTile : a.n;
+    const TileInfo x = a.tinfo[t];
+    const int path = tile_path(x.info);
+    if constexpr (OP == OP_PLAIN) {
+        if (path != TILE_SLOW) wave_plain<T>(a, x, e0, e1, lane);
+        else wave_slow<T, OP>(a, e0, e1, x.p, lane);
+    } else {
+        if (path == TILE_NARROW) wave_narrow<T, OP>(a, x, e0, e1, lane);
+        else if (path == TILE_WIDE) wave_wide<T, OP>(a, x, e0, e1, lane);
+        else wave_slow<T, OP>(a, e0, e1, x.p, lane);
+    }
+}
+
+// Static grid: wave w of workgroup g decodes tile 4g + w.  The hardware
+// workgroup dispatcher hands tiles to CUs as they free up.
+template <class T, int OP>
+__global__ __launch_bounds__(64 * kWavesPerGroup) void k_decode_wave(DecArgs a) {
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * kWavesPerGroup + wave;
+    if (t * kWaveTile >= a.n) return;
+    wave_tile<T, OP>(a, t, lane);
+}
+
+// Work stealing.  The tiles form 8 contiguous queues, one per XCD (the
+// hardware places workgroup b on XCD b % 8).  Every wave the dispatcher
+// launches claims the next tile of its XCD's queue with one atomic; when that
+// queue is drained it marks it exhausted and steals from the other queues.
+// The grid has exactly one wave per tile, so every tile is claimed once.
+// Waves are short-lived on purpose: a persistent wave's next loads would
+// queue behind its own stores (vmcnt is in-order on gfx9), which halves
+// throughput.  The queues are zeroed by a fill kernel before each launch.
+constexpr int kStealQueues = 8;
+constexpr int kQueueStride = 16;  // ints between queue counters (64 B)
+
+__device__ __forceinline__ int64_t steal_claim(int32_t* q, int qi, int64_t per, int64_t ntiles) {
+    for (int tries = 0; tries < kStealQueues; ++tries, qi = (qi + 1) & (kStealQueues - 1)) {
+        int32_t* c = q + qi * kQueueStride;
+        if (__hip_atomic_load(c + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) continue;  // exhausted
+        const int64_t base = qi * per;
+        const int64_t end = (base + per < ntiles) ? base + per : ntiles;
+        int got = 0;
+        if ((threadIdx.x & 63) == 0) got = atomicAdd(c, 1);
+        got = __builtin_amdgcn_readfirstlane(got);
+        if (base + got < end) return base + got;
+        if ((threadIdx.x & 63) == 0) __hip_atomic_store(c + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return -1;
+}
+
+template <class T, int OP>
+__global__ __launch_bounds__(64 * kWavesPerGroup) void k_decode_steal(DecArgs a, int64_t ntiles) {
+    const int lane = threadIdx.x & 63;
+    const int64_t per = (ntiles + kStealQueues - 1) / kStealQueues;
+    const int64_t t = steal_claim(a.sched, static_cast<int>(blockIdx.x & (kStealQueues - 1)), per, ntiles);
+    if (t >= 0) wave_tile<T, OP>(a, t, lane);
+}
+
+__global__ void k_zero_queues(int32_t* q) {
+    if (threadIdx.x < kStealQueues * kQueueStride) q[threadIdx.x] = 0;
+}
+
 template <class F> int dispatch_dtype(int dtype, F&& f) {
     switch (dtype) {
         case GLC_I32: return f(int32_t());
@@ -259,16 +515,30 @@ DecArgs make_args(const glc_column* c, void* out) {
     a.P = c->P;
     a.n_index = c->n_index;
     a.counter = reinterpret_cast<int32_t*>(c->d_tile_part + c->n_index);  // 2 counters after the index
+    a.tinfo = c->d_tinfo;
+    a.wlimit = c->payload_words;
+    a.sched = c->d_sched;
     return a;
 }
 
 template <class T, int OP> int run(const DecArgs& a, int sched, hipStream_t s) {
     const int64_t ntiles = (a.n + kTile - 1) / kTile;
+    const int64_t nwt = (a.n + kWaveTile - 1) / kWaveTile;
     switch (sched) {
         case 0:
-            hipLaunchKernelGGL((k_decode_tiles<T, OP>), dim3(static_cast<unsigned>(ntiles)), dim3(kThreads), 0, s, a);
+            hipLaunchKernelGGL((k_decode_wave<T, OP>), dim3(static_cast<unsigned>((nwt + kWavesPerGroup - 1) / kWavesPerGroup)),
+                               dim3(64 * kWavesPerGroup), 0, s, a);
             break;
         case 1: {
+            hipLaunchKernelGGL(k_zero_queues, dim3(1), dim3(kStealQueues * kQueueStride), 0, s, a.sched);
+            hipLaunchKernelGGL((k_decode_steal<T, OP>), dim3(static_cast<unsigned>((nwt + kWavesPerGroup - 1) / kWavesPerGroup)),
+                               dim3(64 * kWavesPerGroup), 0, s, a, nwt);
+            break;
+        }
+        case 4:
+            hipLaunchKernelGGL((k_decode_tiles<T, OP>), dim3(static_cast<unsigned>(ntiles)), dim3(kThreads), 0, s, a);
+            break;
+        case 5: {
             int64_t g = static_cast<int64_t>(device_cus()) * 8;
             if (g > ntiles) g = ntiles;
             hipLaunchKernelGGL((k_decode_worksteal<T, OP>), dim3(static_cast<unsigned>(g)), dim3(kThreads), 0, s, a,
@@ -310,6 +580,9 @@ int launch_decode_full(const glc_column* c, void* d_out, int mode, hipStream_t s
         case GLC_FULL_WORKSTEAL: plain = c->d_plain != nullptr; sched = 1; break;
         case GLC_FULL_WORKSTEAL_ADV: plain = c->d_plain != nullptr; sched = 3; break;
         case GLC_FULL_COOP_EXACT: plain = true; sched = 1; break;
+        // schedules kept for measurement: LDS-staged workgroup tiles (static / work stealing)
+        case kModeLdsTiles: plain = c->d_plain != nullptr; sched = 4; break;
+        case kModeLdsSteal: plain = c->d_plain != nullptr; sched = 5; break;
         default: return fail(GLC_ERR_INVALID_ARG, "bad full-decompress mode");
     }
     if (plain && !c->d_plain)

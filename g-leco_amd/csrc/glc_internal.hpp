@@ -25,10 +25,13 @@ namespace glc {
 
 constexpr int kIndexShift = 10;                 // tile_part granularity: 1024 elements
 constexpr int64_t kIndexTile = 1LL << kIndexShift;
-constexpr int kPayloadPadWords = 64;            // >= 2 words read past the last needed word, +256 B slack
+constexpr int kPayloadPadWords = 192;           // zero words after the payload: branch-free row loads read
+                                                // up to word base+128 with base clamped to the payload end
 constexpr int kDecodeThreads = 256;             // 4 wave64 per workgroup
 constexpr int kDecodeTile = 2048;               // elements per workgroup tile (8 per thread)
 constexpr int kMaxTileParts = 64;               // partitions of one tile staged in LDS (slow path)
+constexpr int kModeLdsTiles = 100;              // extra full-decode schedules (measurement): LDS workgroup tiles
+constexpr int kModeLdsSteal = 101;              //   ... and their persistent work-stealing form
 
 enum { MODEL_CONSTANT = 0, MODEL_LINEAR = 1, MODEL_POLY2 = 2, MODEL_POLY3 = 3, MODEL_DIRECT_COPY = 4 };
 
@@ -48,6 +51,19 @@ __host__ __device__ inline uint64_t desc_pack(int64_t bit_off, int32_t bits, int
 __host__ __device__ inline int64_t desc_off(uint64_t pk) { return static_cast<int64_t>(pk & ((1ULL << 48) - 1)); }
 __host__ __device__ inline int32_t desc_bits(uint64_t pk) { return static_cast<int32_t>((pk >> 48) & 0xFF); }
 __host__ __device__ inline int32_t desc_type(uint64_t pk) { return static_cast<int32_t>(pk >> 56); }
+
+// Per-1024-element tile of the column: bit position of the tile's first
+// element, the partition holding it, and the decode path for the tile.
+struct alignas(16) TileInfo {
+    int64_t W;     // first bit of element t*1024 (valid when the tile is inside partition p)
+    int32_t p;     // last partition with start <= t*1024
+    int32_t info;  // delta_bits | path << 8
+};
+static_assert(sizeof(TileInfo) == 16, "TileInfo is 16 bytes");
+enum { TILE_SLOW = 0, TILE_NARROW = 1, TILE_WIDE = 2 };
+__host__ __device__ inline int tile_bits(int32_t info) { return info & 0xFF; }
+__host__ __device__ inline int tile_path(int32_t info) { return (info >> 8) & 3; }
+constexpr int kSchedBytes = 1024;  // per-XCD work-stealing queues (16 ints apart) + a done counter
 
 // Summary computed on the device when the index is built (one D2H copy).
 struct IndexSummary {
@@ -88,6 +104,8 @@ struct glc_column {
     glc::Desc* d_desc = nullptr;
     int32_t* d_tile_part = nullptr;
     int64_t n_index = 0;            // entries of tile_part (ceil(N/1024) + 1)
+    glc::TileInfo* d_tinfo = nullptr;  // ceil(N/1024) entries
+    int32_t* d_sched = nullptr;     // work-stealing queues (zeroed; reset by the kernel that uses them)
     uint32_t* d_words = nullptr;    // payload (nullptr after preunpack with free_packed)
     int64_t words_alloc = 0;
     int64_t* d_plain = nullptr;     // pre-unpacked deltas
@@ -114,6 +132,7 @@ int launch_fit(int dtype, const void* d_values, glc_column* c, int32_t bs, hipSt
 int launch_scan_offsets(glc_column* c, int64_t* d_total, hipStream_t s);
 int launch_pack(int dtype, const void* d_values, glc_column* c, hipStream_t s);
 int launch_build_index(glc_column* c, IndexSummary* d_sum, hipStream_t s);
+int launch_build_tinfo(glc_column* c, hipStream_t s);  // after a valid index
 int launch_v6(int dtype, const void* d_values, int64_t n, int32_t base, int32_t* d_start, int32_t* d_end,
               int64_t cap, int64_t* d_count, float* d_var, float* d_thr, hipStream_t s);
 int v6_capacity(int64_t n, int32_t base, int64_t* cap, int64_t* nvb);
