@@ -38,6 +38,44 @@ struct RaArgs {
     int32_t fixed;  // S for the fixed lookup, 0 = generic
 };
 
+// Value at element e of a column (0 outside [0, N) and in gaps).
+template <class T, bool PLAIN>
+__device__ __forceinline__ T ra_lookup(const Desc* __restrict__ desc, const int32_t* __restrict__ tile_part,
+                                       const uint32_t* __restrict__ words, const int64_t* __restrict__ plain,
+                                       const double* __restrict__ params, int64_t n, int32_t fixed, int64_t e) {
+    if (e < 0 || e >= n) return T(0);
+    int64_t p;
+    if (fixed > 0) {
+        p = e / fixed;
+    } else {
+        const int64_t t = e >> kIndexShift;
+        p = tile_part[t];
+        int64_t hi = tile_part[t + 1];
+        while (p < hi) {
+            const int64_t m = (p + hi + 1) >> 1;
+            if (desc[m].start <= e) p = m; else hi = m - 1;
+        }
+    }
+    const Desc d = desc[p];
+    const int64_t k = e - d.start;
+    if (k < 0 || k >= d.len) return T(0);
+    const int32_t type = desc_type(d.packed);
+    const int b = desc_bits(d.packed);
+    const int32_t i = static_cast<int32_t>(k);
+    if constexpr (PLAIN) {
+        const int64_t pl = plain[e];
+        if (type == MODEL_DIRECT_COPY) return static_cast<T>(pl);
+        if (type == MODEL_POLY2) return apply_delta<T>(dec_pred_poly<T>(type, d.theta0, d.theta1, params[4 * p + 2], i), pl);
+        return apply_delta<T>(dec_pred<T>(d.theta0, d.theta1, i), pl);
+    } else {
+        const uint64_t raw = (b > 0) ? extract_raw_g(words, desc_off(d.packed) + k * b, b) : 0;
+        if (type == MODEL_DIRECT_COPY) return static_cast<T>(raw);
+        if (type == MODEL_POLY2)
+            return apply_delta<T>(dec_pred_poly<T>(type, d.theta0, d.theta1, params[4 * p + 2], i), sign_extend(raw, b));
+        return apply_delta<T>(dec_pred<T>(d.theta0, d.theta1, i), sign_extend(raw, b));
+    }
+}
+
 template <class T, bool POS64, bool PLAIN>
 __global__ __launch_bounds__(kThreads) void k_random_access(RaArgs a) {
     T* __restrict__ out = static_cast<T*>(a.out);
@@ -45,43 +83,41 @@ __global__ __launch_bounds__(kThreads) void k_random_access(RaArgs a) {
     for (int64_t j = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x; j < a.q; j += stride) {
         const int64_t e = POS64 ? static_cast<const int64_t*>(a.pos)[j]
                                 : static_cast<int64_t>(static_cast<const int32_t*>(a.pos)[j]);
-        T v = T(0);
-        if (e >= 0 && e < a.n) {
-            int64_t p;
-            if (a.fixed > 0) {
-                p = e / a.fixed;
-            } else {
-                const int64_t t = e >> kIndexShift;
-                p = a.tile_part[t];
-                int64_t hi = a.tile_part[t + 1];
-                while (p < hi) {
-                    const int64_t m = (p + hi + 1) >> 1;
-                    if (a.desc[m].start <= e) p = m; else hi = m - 1;
-                }
-            }
-            const Desc d = a.desc[p];
-            const int64_t k = e - d.start;
-            if (k >= 0 && k < d.len) {
-                const int32_t type = desc_type(d.packed);
-                const int b = desc_bits(d.packed);
-                const int32_t i = static_cast<int32_t>(k);
-                if constexpr (PLAIN) {
-                    const int64_t pl = a.plain[e];
-                    if (type == MODEL_DIRECT_COPY) v = static_cast<T>(pl);
-                    else if (type == MODEL_POLY2)
-                        v = apply_delta<T>(dec_pred_poly<T>(type, d.theta0, d.theta1, a.params[4 * p + 2], i), pl);
-                    else v = apply_delta<T>(dec_pred<T>(d.theta0, d.theta1, i), pl);
-                } else {
-                    const uint64_t raw = (b > 0) ? extract_raw_g(a.words, desc_off(d.packed) + k * b, b) : 0;
-                    if (type == MODEL_DIRECT_COPY) v = static_cast<T>(raw);
-                    else if (type == MODEL_POLY2)
-                        v = apply_delta<T>(dec_pred_poly<T>(type, d.theta0, d.theta1, a.params[4 * p + 2], i),
-                                           sign_extend(raw, b));
-                    else v = apply_delta<T>(dec_pred<T>(d.theta0, d.theta1, i), sign_extend(raw, b));
-                }
-            }
+        out[j] = ra_lookup<T, PLAIN>(a.desc, a.tile_part, a.words, a.plain, a.params, a.n, a.fixed, e);
+    }
+}
+
+// Lookups of many requests over many columns in one launch: lookup j
+// belongs to the request r with reqs[r].first <= j < reqs[r+1].first.
+__global__ __launch_bounds__(kThreads) void k_serve_lookup(const ServeReq* __restrict__ reqs, int32_t R, int64_t Q) {
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * kThreads;
+    for (int64_t j = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x; j < Q; j += stride) {
+        int32_t lo = 0, hi = R - 1;
+        while (lo < hi) {
+            const int32_t m = (lo + hi + 1) >> 1;
+            if (reqs[m].first <= j) lo = m; else hi = m - 1;
         }
-        out[j] = v;
+        const ServeReq& r = reqs[lo];
+        const int64_t k = j - r.first;
+        const int64_t e = r.pos[k];
+        switch (r.dtype) {
+            case GLC_I32:
+                static_cast<int32_t*>(r.out)[k] =
+                    ra_lookup<int32_t, false>(r.desc, r.tile_part, r.words, nullptr, r.params, r.n, r.fixed, e);
+                break;
+            case GLC_U32:
+                static_cast<uint32_t*>(r.out)[k] =
+                    ra_lookup<uint32_t, false>(r.desc, r.tile_part, r.words, nullptr, r.params, r.n, r.fixed, e);
+                break;
+            case GLC_I64:
+                static_cast<int64_t*>(r.out)[k] =
+                    ra_lookup<int64_t, false>(r.desc, r.tile_part, r.words, nullptr, r.params, r.n, r.fixed, e);
+                break;
+            default:
+                static_cast<uint64_t*>(r.out)[k] =
+                    ra_lookup<uint64_t, false>(r.desc, r.tile_part, r.words, nullptr, r.params, r.n, r.fixed, e);
+                break;
+        }
     }
 }
 
@@ -135,6 +171,13 @@ unsigned ra_grid(int64_t q) {
 }
 
 }  // namespace
+
+int launch_serve_lookup(const ServeReq* d_reqs, int32_t R, int64_t Q, hipStream_t s) {
+    if (Q <= 0 || R <= 0) return GLC_OK;
+    hipLaunchKernelGGL(k_serve_lookup, dim3(ra_grid(Q)), dim3(kThreads), 0, s, d_reqs, R, Q);
+    GLC_HIP(hipGetLastError());
+    return GLC_OK;
+}
 
 int launch_random_access(const glc_column* c, const void* d_pos, int pos64, int64_t q, void* d_out, int mode,
                          int32_t fixed_size, hipStream_t s) {

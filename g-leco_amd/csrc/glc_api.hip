@@ -24,15 +24,6 @@
 
 namespace glc {
 
-namespace {
-thread_local std::string g_last_error;
-
-int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
-
-struct ArenaLayout {
-    int64_t start, end, type, bits, params, off, err, desc, tile, tinfo, sched, summary, total;
-};
-
 ArenaLayout layout_for(int64_t P, int64_t n_index) {
     ArenaLayout L;
     int64_t cur = 0;
@@ -57,18 +48,9 @@ ArenaLayout layout_for(int64_t P, int64_t n_index) {
     return L;
 }
 
-int alloc_column(glc_column* c, int64_t P, hipStream_t s) {
-    c->P = P;
-    c->n_index = (c->n + kIndexTile - 1) / kIndexTile + 1;
-    ArenaLayout L = layout_for(P, c->n_index);
-    void* arena = nullptr;
-    if (hipMalloc(&arena, L.total) != hipSuccess) {
-        (void)hipGetLastError();
-        return fail(GLC_ERR_OOM, "device allocation of column metadata failed (" + std::to_string(L.total) + " B)");
-    }
-    c->meta_arena = arena;
+void bind_arena(glc_column* c, char* b, const ArenaLayout& L) {
+    c->meta_arena = b;
     c->meta_bytes = L.total;
-    char* b = static_cast<char*>(arena);
     c->d_start = reinterpret_cast<int32_t*>(b + L.start);
     c->d_end = reinterpret_cast<int32_t*>(b + L.end);
     c->d_type = reinterpret_cast<int32_t*>(b + L.type);
@@ -80,14 +62,37 @@ int alloc_column(glc_column* c, int64_t P, hipStream_t s) {
     c->d_tile_part = reinterpret_cast<int32_t*>(b + L.tile);
     c->d_tinfo = reinterpret_cast<TileInfo*>(b + L.tinfo);
     c->d_sched = reinterpret_cast<int32_t*>(b + L.sched);
-    GLC_HIP(hipMemsetAsync(arena, 0, L.total, s));
-    return GLC_OK;
 }
 
 IndexSummary* summary_ptr(glc_column* c) {
     ArenaLayout L = layout_for(c->P, c->n_index);
     return reinterpret_cast<IndexSummary*>(static_cast<char*>(c->meta_arena) + L.summary);
 }
+
+// api/G-LeCo.cuh:519-520
+int32_t fit_block_size(int64_t len0) {
+    int64_t r = (len0 + 31) / 32 * 32;
+    int64_t b = std::min<int64_t>(256, r);
+    return static_cast<int32_t>(std::max<int64_t>(b, 32));
+}
+
+namespace {
+thread_local std::string g_last_error;
+
+int alloc_column(glc_column* c, int64_t P, hipStream_t s) {
+    c->P = P;
+    c->n_index = (c->n + kIndexTile - 1) / kIndexTile + 1;
+    ArenaLayout L = layout_for(P, c->n_index);
+    void* arena = nullptr;
+    if (hipMalloc(&arena, L.total) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(GLC_ERR_OOM, "device allocation of column metadata failed (" + std::to_string(L.total) + " B)");
+    }
+    bind_arena(c, static_cast<char*>(arena), L);
+    GLC_HIP(hipMemsetAsync(arena, 0, L.total, s));
+    return GLC_OK;
+}
+
 
 int alloc_payload(glc_column* c, int64_t payload_words, hipStream_t s) {
     c->payload_words = payload_words;
@@ -104,7 +109,7 @@ int alloc_payload(glc_column* c, int64_t payload_words, hipStream_t s) {
 }
 
 void destroy(glc_column* c) {
-    if (!c) return;
+    if (!c || c->view) return;
     if (c->meta_arena) (void)hipFree(c->meta_arena);
     if (c->d_words) (void)hipFree(c->d_words);
     if (c->d_plain) (void)hipFree(c->d_plain);
@@ -128,12 +133,6 @@ int finish_index(glc_column* c, hipStream_t s, IndexSummary* out) {
     return launch_build_tinfo(c, s);
 }
 
-// api/G-LeCo.cuh:519-520
-int32_t fit_block_size(int64_t len0) {
-    int64_t r = (len0 + 31) / 32 * 32;
-    int64_t b = std::min<int64_t>(256, r);
-    return static_cast<int32_t>(std::max<int64_t>(b, 32));
-}
 
 __global__ void k_fixed_parts(int32_t* st, int32_t* en, int64_t P, int64_t n, int32_t S) {
     const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;

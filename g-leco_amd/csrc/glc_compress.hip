@@ -30,97 +30,126 @@ constexpr int kFitThreads = 256;
 // reference does (:113-137), so theta0/theta1 are bit-identical for the same
 // contraction form (fma in sum_xx, sum_xy, det, numerator and theta0).
 // ---------------------------------------------------------------------------
+struct FitSmem {
+    double sum[4][kFitThreads];
+    double theta[2];
+    int64_t mx[kFitThreads / 64];
+};
+
+// Partition p of n values at x (n > 0).  Called by the whole workgroup.
+template <class T>
+__device__ __forceinline__ void fit_one(const T* __restrict__ x, int32_t n, int bs, int64_t p, int32_t* __restrict__ type,
+                                        double* __restrict__ params, int32_t* __restrict__ bits,
+                                        int64_t* __restrict__ err, FitSmem& sm) {
+    const int tid = threadIdx.x;
+    double a = 0.0, b = 0.0, c = 0.0, d = 0.0;
+    int ovf = 0;
+    if (tid < bs) {
+        for (int32_t i = tid; i < n; i += bs) {
+            const T val = x[i];
+            ovf |= might_overflow_double<T>(val) ? 1 : 0;
+            const double xi = static_cast<double>(i);
+            const double yi = static_cast<double>(val);
+            a = a + xi;
+            b = b + yi;
+            c = __builtin_fma(xi, xi, c);
+            d = __builtin_fma(xi, yi, d);
+        }
+    }
+    if (__syncthreads_or(ovf)) {  // :95-106 DIRECT_COPY
+        if (tid == 0) {
+            type[p] = MODEL_DIRECT_COPY;
+            params[4 * p] = 0.0; params[4 * p + 1] = 0.0; params[4 * p + 2] = 0.0; params[4 * p + 3] = 0.0;
+            err[p] = 0;
+            bits[p] = static_cast<int32_t>(sizeof(T) * 8);
+        }
+        __syncthreads();
+        return;
+    }
+    sm.sum[0][tid] = a; sm.sum[1][tid] = b; sm.sum[2][tid] = c; sm.sum[3][tid] = d;
+    __syncthreads();
+    for (int h = bs >> 1; h > 0; h >>= 1) {
+        if (tid < h) {
+            sm.sum[0][tid] = sm.sum[0][tid] + sm.sum[0][tid + h];
+            sm.sum[1][tid] = sm.sum[1][tid] + sm.sum[1][tid + h];
+            sm.sum[2][tid] = sm.sum[2][tid] + sm.sum[2][tid + h];
+            sm.sum[3][tid] = sm.sum[3][tid] + sm.sum[3][tid + h];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        const double dn = static_cast<double>(n);
+        const double Sx = sm.sum[0][0], Sy = sm.sum[1][0], Sxx = sm.sum[2][0], Sxy = sm.sum[3][0];
+        const double det = __builtin_fma(dn, Sxx, -(Sx * Sx));
+        double t0, t1;
+        if (__builtin_fabs(det) > 1e-10) {
+            t1 = __builtin_fma(dn, Sxy, -(Sx * Sy)) / det;
+            t0 = __builtin_fma(-t1, Sx, Sy) / dn;
+        } else {
+            t1 = 0.0;
+            t0 = Sy / dn;
+        }
+        sm.theta[0] = t0; sm.theta[1] = t1;
+        type[p] = MODEL_LINEAR;
+        params[4 * p] = t0; params[4 * p + 1] = t1; params[4 * p + 2] = 0.0; params[4 * p + 3] = 0.0;
+    }
+    __syncthreads();
+    const double t0 = sm.theta[0], t1 = sm.theta[1];
+    int64_t mx = 0;
+    for (int32_t i = tid; i < n; i += kFitThreads) {
+        const int64_t dl = calc_delta<T>(x[i], enc_pred<T>(t0, t1, i));
+        const int64_t ab = dl < 0 ? static_cast<int64_t>(0ULL - static_cast<uint64_t>(dl)) : dl;
+        mx = ab > mx ? ab : mx;  // -LLONG_MIN stays negative and is ignored, as in :171-174
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const int64_t y = __shfl_xor(mx, o, 64);
+        mx = y > mx ? y : mx;
+    }
+    if ((tid & 63) == 0) sm.mx[tid >> 6] = mx;
+    __syncthreads();
+    if (tid == 0) {
+        int64_t m = sm.mx[0];
+        for (int w = 1; w < kFitThreads / 64; ++w) m = sm.mx[w] > m ? sm.mx[w] : m;
+        err[p] = m;
+        int32_t bb = 0;
+        if (m > 0) {
+            bb = 64 - __builtin_clzll(static_cast<unsigned long long>(m)) + 1;
+            bb = bb > 64 ? 64 : bb;
+        }
+        bits[p] = bb;
+    }
+    __syncthreads();
+}
+
 template <class T>
 __global__ __launch_bounds__(kFitThreads) void k_fit(const T* __restrict__ v, const int32_t* __restrict__ st,
                                                      const int32_t* __restrict__ en, int32_t* __restrict__ type,
                                                      double* __restrict__ params, int32_t* __restrict__ bits,
                                                      int64_t* __restrict__ err, int64_t P, int bs) {
-    __shared__ double s_sum[4][kFitThreads];
-    __shared__ double s_theta[2];
-    __shared__ int64_t s_max[kFitThreads / 64];
-    const int tid = threadIdx.x;
+    __shared__ FitSmem sm;
     for (int64_t p = blockIdx.x; p < P; p += gridDim.x) {
         const int32_t s0 = st[p];
         const int32_t n = en[p] - s0;
         if (n <= 0) continue;  // :62, entry keeps its initial values
-        const T* __restrict__ x = v + s0;
-        double a = 0.0, b = 0.0, c = 0.0, d = 0.0;
-        int ovf = 0;
-        if (tid < bs) {
-            for (int32_t i = tid; i < n; i += bs) {
-                const T val = x[i];
-                ovf |= might_overflow_double<T>(val) ? 1 : 0;
-                const double xi = static_cast<double>(i);
-                const double yi = static_cast<double>(val);
-                a = a + xi;
-                b = b + yi;
-                c = __builtin_fma(xi, xi, c);
-                d = __builtin_fma(xi, yi, d);
-            }
-        }
-        if (__syncthreads_or(ovf)) {  // :95-106 DIRECT_COPY
-            if (tid == 0) {
-                type[p] = MODEL_DIRECT_COPY;
-                params[4 * p] = 0.0; params[4 * p + 1] = 0.0; params[4 * p + 2] = 0.0; params[4 * p + 3] = 0.0;
-                err[p] = 0;
-                bits[p] = static_cast<int32_t>(sizeof(T) * 8);
-            }
-            __syncthreads();
-            continue;
-        }
-        s_sum[0][tid] = a; s_sum[1][tid] = b; s_sum[2][tid] = c; s_sum[3][tid] = d;
-        __syncthreads();
-        for (int h = bs >> 1; h > 0; h >>= 1) {
-            if (tid < h) {
-                s_sum[0][tid] = s_sum[0][tid] + s_sum[0][tid + h];
-                s_sum[1][tid] = s_sum[1][tid] + s_sum[1][tid + h];
-                s_sum[2][tid] = s_sum[2][tid] + s_sum[2][tid + h];
-                s_sum[3][tid] = s_sum[3][tid] + s_sum[3][tid + h];
-            }
-            __syncthreads();
-        }
-        if (tid == 0) {
-            const double dn = static_cast<double>(n);
-            const double Sx = s_sum[0][0], Sy = s_sum[1][0], Sxx = s_sum[2][0], Sxy = s_sum[3][0];
-            const double det = __builtin_fma(dn, Sxx, -(Sx * Sx));
-            double t0, t1;
-            if (__builtin_fabs(det) > 1e-10) {
-                t1 = __builtin_fma(dn, Sxy, -(Sx * Sy)) / det;
-                t0 = __builtin_fma(-t1, Sx, Sy) / dn;
-            } else {
-                t1 = 0.0;
-                t0 = Sy / dn;
-            }
-            s_theta[0] = t0; s_theta[1] = t1;
-            type[p] = MODEL_LINEAR;
-            params[4 * p] = t0; params[4 * p + 1] = t1; params[4 * p + 2] = 0.0; params[4 * p + 3] = 0.0;
-        }
-        __syncthreads();
-        const double t0 = s_theta[0], t1 = s_theta[1];
-        int64_t mx = 0;
-        for (int32_t i = tid; i < n; i += kFitThreads) {
-            const int64_t dl = calc_delta<T>(x[i], enc_pred<T>(t0, t1, i));
-            const int64_t ab = dl < 0 ? static_cast<int64_t>(0ULL - static_cast<uint64_t>(dl)) : dl;
-            mx = ab > mx ? ab : mx;  // -LLONG_MIN stays negative and is ignored, as in :171-174
-        }
-        for (int o = 32; o > 0; o >>= 1) {
-            const int64_t y = __shfl_xor(mx, o, 64);
-            mx = y > mx ? y : mx;
-        }
-        if ((tid & 63) == 0) s_max[tid >> 6] = mx;
-        __syncthreads();
-        if (tid == 0) {
-            int64_t m = s_max[0];
-            for (int w = 1; w < kFitThreads / 64; ++w) m = s_max[w] > m ? s_max[w] : m;
-            err[p] = m;
-            int32_t bb = 0;
-            if (m > 0) {
-                bb = 64 - __builtin_clzll(static_cast<unsigned long long>(m)) + 1;
-                bb = bb > 64 ? 64 : bb;
-            }
-            bits[p] = bb;
-        }
-        __syncthreads();
+        fit_one<T>(v + s0, n, bs, p, type, params, bits, err, sm);
+    }
+}
+
+// Batched fit: global partition g belongs to column part_col[g]; only the
+// columns of element type T are visited (one launch per type present).
+template <class T>
+__global__ __launch_bounds__(kFitThreads) void k_fit_batch(const BatchCol* __restrict__ cols,
+                                                           const int32_t* __restrict__ part_col, int64_t Ptot,
+                                                           int dtype) {
+    __shared__ FitSmem sm;
+    for (int64_t g = blockIdx.x; g < Ptot; g += gridDim.x) {
+        const BatchCol& c = cols[part_col[g]];
+        if (c.dtype != dtype) continue;
+        const int64_t p = g - c.part_base;
+        const int32_t s0 = c.start[p];
+        const int32_t n = c.end[p] - s0;
+        if (n <= 0) continue;
+        fit_one<T>(static_cast<const T*>(c.values) + s0, n, c.fit_bs, p, c.type, c.params, c.bits, c.err, sm);
     }
 }
 
@@ -162,6 +191,36 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(F f, int64_t P, int64_t* 
         acc += f(p);
     }
     if (tid == kScanThreads - 1) *total = s[tid];
+}
+
+// Batched offsets: one workgroup per column scans that column's partitions
+// and reserves the column's payload words in *col_words.
+__global__ __launch_bounds__(kScanThreads) void k_scan_batch(const BatchCol* __restrict__ cols,
+                                                             int64_t* __restrict__ col_bits) {
+    const BatchCol& c = cols[blockIdx.x];
+    LenBits f{c.start, c.end, c.bits};
+    __shared__ int64_t s[kScanThreads];
+    const int tid = threadIdx.x;
+    const int64_t P = c.P;
+    const int64_t chunk = (P + kScanThreads - 1) / kScanThreads;
+    const int64_t b0 = tid * chunk;
+    const int64_t b1 = (b0 + chunk < P) ? b0 + chunk : P;
+    int64_t local = 0;
+    for (int64_t p = b0; p < b1; ++p) local += f(p);
+    s[tid] = local;
+    __syncthreads();
+    for (int o = 1; o < kScanThreads; o <<= 1) {
+        int64_t y = (tid >= o) ? s[tid - o] : 0;
+        __syncthreads();
+        s[tid] += y;
+        __syncthreads();
+    }
+    int64_t acc = s[tid] - local;
+    for (int64_t p = b0; p < b1; ++p) {
+        c.off[p] = acc;
+        acc += f(p);
+    }
+    if (tid == kScanThreads - 1) col_bits[blockIdx.x] = s[tid];
 }
 
 // ---------------------------------------------------------------------------
@@ -228,6 +287,52 @@ __global__ void k_build_tinfo(const Desc* __restrict__ desc, const int32_t* __re
     ti[t] = x;
 }
 
+// Batched index: descriptors + tile_part for every partition of every column
+// (fixed partitions, so always well formed).
+__global__ void k_build_index_batch(const BatchCol* __restrict__ cols, const int32_t* __restrict__ part_col,
+                                    int64_t Ptot) {
+    const int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (g >= Ptot) return;
+    const BatchCol& c = cols[part_col[g]];
+    const int64_t p = g - c.part_base;
+    const int32_t s = c.start[p], e = c.end[p];
+    Desc d;
+    d.theta0 = c.params[4 * p];
+    d.theta1 = c.params[4 * p + 1];
+    d.packed = desc_pack(c.off[p], c.bits[p], c.type[p]);
+    d.start = s;
+    d.len = e > s ? e - s : 0;
+    c.desc[p] = d;
+    atomicMax(&c.sum->max_bits, c.bits[p]);
+    const int32_t s_next = (p + 1 < c.P) ? c.start[p + 1] : 0x7FFFFFFF;
+    const int64_t t0 = (p == 0) ? 0 : (static_cast<int64_t>(s) + kIndexTile - 1) >> kIndexShift;
+    int64_t t1 = (p + 1 < c.P) ? (static_cast<int64_t>(s_next) + kIndexTile - 1) >> kIndexShift : c.n_index;
+    if (t1 > c.n_index) t1 = c.n_index;
+    for (int64_t t = t0; t < t1; ++t) c.tile_part[t] = static_cast<int32_t>(p);
+}
+
+__global__ void k_build_tinfo_batch(const BatchCol* __restrict__ cols, const int32_t* __restrict__ tile_col,
+                                    int64_t Ttot) {
+    const int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (g >= Ttot) return;
+    const BatchCol& c = cols[tile_col[g]];
+    const int64_t t = g - c.tile_base;
+    const int64_t e0 = t * kIndexTile;
+    const int64_t e1 = (e0 + kIndexTile < c.n) ? e0 + kIndexTile : c.n;
+    const int32_t p = c.tile_part[t];
+    const Desc d = c.desc[p];
+    const int b = desc_bits(d.packed);
+    const int type = desc_type(d.packed);
+    const bool inside = e0 >= d.start && e1 <= static_cast<int64_t>(d.start) + d.len;
+    int path = TILE_SLOW;
+    if (inside && type != MODEL_POLY2) path = (b <= 32 && type != MODEL_DIRECT_COPY) ? TILE_NARROW : TILE_WIDE;
+    TileInfo x;
+    x.W = inside ? desc_off(d.packed) + (e0 - d.start) * b : 0;
+    x.p = p;
+    x.info = b | (path << 8);
+    c.tinfo[t] = x;
+}
+
 // ---------------------------------------------------------------------------
 // Pack.  One workgroup per 2048-element tile.
 // ---------------------------------------------------------------------------
@@ -251,25 +356,21 @@ __device__ __forceinline__ int64_t bitpos_lb(const Desc* __restrict__ desc, int6
     return desc_off(d.packed) + k * desc_bits(d.packed);
 }
 
+// Pack elements [e0, e1) of one column into its payload (zeroed beforehand).
 template <class T>
-__global__ __launch_bounds__(kPackThreads) void k_pack(const T* __restrict__ v, const Desc* __restrict__ desc,
-                                                       const int32_t* __restrict__ tile_part, int64_t n,
-                                                       uint32_t* __restrict__ words) {
-    __shared__ uint32_t s_w[kPackWords];
-    __shared__ int64_t s_w0, s_w1;
+__device__ __forceinline__ void pack_tile(const T* __restrict__ v, const Desc* __restrict__ desc,
+                                          const int32_t* __restrict__ tile_part, int64_t e0, int64_t e1,
+                                          uint32_t* __restrict__ words, uint32_t* s_w, int64_t* s_ww) {
     const int tid = threadIdx.x;
-    const int64_t e0 = static_cast<int64_t>(blockIdx.x) * kPackTile;
-    if (e0 >= n) return;
-    const int64_t e1 = (e0 + kPackTile < n) ? e0 + kPackTile : n;
     const int64_t plo = tile_part[e0 >> kIndexShift];
     const int64_t phi = tile_part[(e1 + kIndexTile - 1) >> kIndexShift];
     if (tid == 0) {
-        s_w0 = bitpos_lb(desc, plo, phi, e0);
-        s_w1 = bitpos_lb(desc, plo, phi, e1 - 1);
+        s_ww[0] = bitpos_lb(desc, plo, phi, e0);
+        s_ww[1] = bitpos_lb(desc, plo, phi, e1 - 1);
     }
     for (int i = tid; i < kPackWords; i += kPackThreads) s_w[i] = 0u;
     __syncthreads();
-    const int64_t wbase = s_w0 >> 5;  // first word touched by this tile
+    const int64_t wbase = s_ww[0] >> 5;  // first word touched by this tile
     for (int64_t e = e0 + tid; e < e1; e += kPackThreads) {
         int64_t lo = plo, hi = phi;
         while (lo < hi) {
@@ -297,7 +398,7 @@ __global__ __launch_bounds__(kPackThreads) void k_pack(const T* __restrict__ v, 
     }
     __syncthreads();
     // words [wbase, wlast]; the first and last may be shared with neighbours
-    const int64_t wlast = (s_w1 >= 0) ? (s_w1 + 64) >> 5 : wbase;  // conservative: covers the last element
+    const int64_t wlast = (s_ww[1] >= 0) ? (s_ww[1] + 64) >> 5 : wbase;  // covers the last element
     const int64_t nw = wlast - wbase + 1;
     for (int64_t k = tid; k < nw && k < kPackWords; k += kPackThreads) {
         const uint32_t val = s_w[k];
@@ -306,6 +407,35 @@ __global__ __launch_bounds__(kPackThreads) void k_pack(const T* __restrict__ v, 
         } else {
             words[wbase + k] = val;
         }
+    }
+    __syncthreads();  // LDS reused by the next tile
+}
+
+template <class T>
+__global__ __launch_bounds__(kPackThreads) void k_pack(const T* __restrict__ v, const Desc* __restrict__ desc,
+                                                       const int32_t* __restrict__ tile_part, int64_t n,
+                                                       uint32_t* __restrict__ words) {
+    __shared__ uint32_t s_w[kPackWords];
+    __shared__ int64_t s_ww[2];
+    const int64_t e0 = static_cast<int64_t>(blockIdx.x) * kPackTile;
+    if (e0 >= n) return;
+    const int64_t e1 = (e0 + kPackTile < n) ? e0 + kPackTile : n;
+    pack_tile<T>(v, desc, tile_part, e0, e1, words, s_w, s_ww);
+}
+
+// Batched pack over 1024-element tiles of every column of type T.
+template <class T>
+__global__ __launch_bounds__(kPackThreads) void k_pack_batch(const BatchCol* __restrict__ cols,
+                                                             const int32_t* __restrict__ tile_col, int64_t Ttot,
+                                                             int dtype) {
+    __shared__ uint32_t s_w[kPackWords];
+    __shared__ int64_t s_ww[2];
+    for (int64_t g = blockIdx.x; g < Ttot; g += gridDim.x) {
+        const BatchCol& c = cols[tile_col[g]];
+        if (c.dtype != dtype || c.payload_words > c.words_reserved) continue;
+        const int64_t e0 = (g - c.tile_base) * kIndexTile;
+        const int64_t e1 = (e0 + kIndexTile < c.n) ? e0 + kIndexTile : c.n;
+        pack_tile<T>(static_cast<const T*>(c.values), c.desc, c.tile_part, e0, e1, c.words, s_w, s_ww);
     }
 }
 
@@ -539,6 +669,94 @@ int launch_v6(int dtype, const void* d_values, int64_t n, int32_t base, int32_t*
     GLC_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_v6_emit, dim3(grid_for(nvb, 256)), dim3(256), 0, s, f, nvb, d_base, d_start, d_end);
     GLC_HIP(hipGetLastError());
+    return GLC_OK;
+}
+
+__global__ void k_fixed_parts_batch(const BatchCol* __restrict__ cols, const int32_t* __restrict__ part_col,
+                                    int64_t Ptot, int32_t S) {
+    const int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (g >= Ptot) return;
+    const BatchCol& c = cols[part_col[g]];
+    const int64_t p = g - c.part_base;
+    const int64_t s0 = p * S;
+    c.start[p] = static_cast<int32_t>(s0);
+    c.end[p] = static_cast<int32_t>((s0 + S < c.n) ? s0 + S : c.n);
+    c.type[p] = MODEL_LINEAR;  // host PartitionInfo init, G:409-414
+}
+
+__global__ void k_batch_prologue(const BatchCol* __restrict__ cols, int C) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < C) *cols[i].sum = IndexSummary{};
+}
+
+// Zero the payload words a 1024-element tile shares with its neighbours (its
+// first word and last three, the ones k_pack_batch ORs into); every other
+// payload word is written whole by exactly one tile.
+__global__ void k_zero_edges_batch(const BatchCol* __restrict__ cols, const int32_t* __restrict__ tile_col,
+                                   int64_t Ttot) {
+    const int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (g >= Ttot) return;
+    const BatchCol& c = cols[tile_col[g]];
+    if (c.payload_words > c.words_reserved) return;
+    const int64_t t = g - c.tile_base;
+    const int64_t e0 = t * kIndexTile;
+    const int64_t e1 = (e0 + kIndexTile < c.n) ? e0 + kIndexTile : c.n;
+    const int64_t plo = c.tile_part[e0 >> kIndexShift];
+    const int64_t phi = c.tile_part[(e1 + kIndexTile - 1) >> kIndexShift];
+    const int64_t wbase = bitpos_lb(c.desc, plo, phi, e0) >> 5;
+    const int64_t last = bitpos_lb(c.desc, plo, phi, e1 - 1);
+    const int64_t wlast = (last >= 0) ? (last + 64) >> 5 : wbase;
+    c.words[wbase] = 0u;
+    for (int64_t w = (wlast - 2 > wbase) ? wlast - 2 : wbase + 1; w <= wlast; ++w) c.words[w] = 0u;
+}
+
+__global__ void k_set_payload_words(BatchCol* __restrict__ cols, const int64_t* __restrict__ col_bits, int C) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= C) return;
+    const int64_t w = (col_bits[i] + 31) / 32;
+    cols[i].payload_words = w;
+    if (w > cols[i].words_reserved) cols[i].sum->bad_bits = 1;  // cannot happen: reserve is the worst case
+}
+
+int launch_batch_compress(const BatchPlan& b, hipStream_t s) {
+    if (b.Ptot == 0) return GLC_OK;
+    hipLaunchKernelGGL(k_batch_prologue, dim3(grid_for(b.C, 256)), dim3(256), 0, s, b.cols, b.C);
+    GLC_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_fixed_parts_batch, dim3(grid_for(b.Ptot, 256)), dim3(256), 0, s, b.cols, b.part_col, b.Ptot,
+                       b.S);
+    GLC_HIP(hipGetLastError());
+    const unsigned gf = static_cast<unsigned>(b.Ptot < (1LL << 20) ? b.Ptot : (1LL << 20));
+    for (int dt = GLC_I32; dt <= GLC_U64; ++dt) {
+        if (!b.has_dtype[dt]) continue;
+        int rc = dispatch_dtype(dt, [&](auto t) {
+            using T = decltype(t);
+            hipLaunchKernelGGL(k_fit_batch<T>, dim3(gf), dim3(kFitThreads), 0, s, b.cols, b.part_col, b.Ptot, dt);
+            GLC_HIP(hipGetLastError());
+            return GLC_OK;
+        });
+        if (rc) return rc;
+    }
+    hipLaunchKernelGGL(k_scan_batch, dim3(static_cast<unsigned>(b.C)), dim3(kScanThreads), 0, s, b.cols, b.col_bits);
+    GLC_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_build_index_batch, dim3(grid_for(b.Ptot, 256)), dim3(256), 0, s, b.cols, b.part_col, b.Ptot);
+    GLC_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_build_tinfo_batch, dim3(grid_for(b.Ttot, 256)), dim3(256), 0, s, b.cols, b.tile_col, b.Ttot);
+    GLC_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_set_payload_words, dim3(grid_for(b.C, 256)), dim3(256), 0, s, b.cols, b.col_bits, b.C);
+    GLC_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_zero_edges_batch, dim3(grid_for(b.Ttot, 256)), dim3(256), 0, s, b.cols, b.tile_col, b.Ttot);
+    GLC_HIP(hipGetLastError());
+    const unsigned gp = static_cast<unsigned>(b.Ttot < (1LL << 20) ? b.Ttot : (1LL << 20));
+    for (int dt = GLC_I32; dt <= GLC_U64; ++dt) {
+        if (!b.has_dtype[dt]) continue;
+        int rc = dispatch_dtype(dt, [&](auto t) {
+            using T = decltype(t);
+            hipLaunchKernelGGL(k_pack_batch<T>, dim3(gp), dim3(kPackThreads), 0, s, b.cols, b.tile_col, b.Ttot, dt);
+            GLC_HIP(hipGetLastError());
+            return GLC_OK;
+        });
+        if (rc) return rc;
+    }
     return GLC_OK;
 }
 

@@ -481,6 +481,44 @@ __global__ void k_zero_queues(int32_t* q) {
     if (threadIdx.x < kStealQueues * kQueueStride) q[threadIdx.x] = 0;
 }
 
+// Batched decode: wave (w, k) of workgroup g decodes global tile
+// (g*WPG + w)*TPW + k of the batch; the column comes from the tile map and the
+// column table (scalar loads), and the element type selects the code path.
+template <int WPG, int TPW>
+__global__ __launch_bounds__(64 * WPG) void k_batch_decode(const BatchCol* __restrict__ cols,
+                                                           const int32_t* __restrict__ tile_col,
+                                                           void* const* __restrict__ outs, int64_t Ttot) {
+    const int lane = threadIdx.x & 63;
+    const int wave = (WPG == 1) ? 0 : __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    for (int k = 0; k < TPW; ++k) {
+        const int64_t g = (static_cast<int64_t>(blockIdx.x) * WPG + wave) * TPW + k;
+        if (g >= Ttot) return;
+        const int32_t ci = tile_col[g];
+        const BatchCol& c = cols[ci];
+        DecArgs a;
+        a.desc = c.desc;
+        a.tile_part = c.tile_part;
+        a.words = c.words;
+        a.plain = nullptr;
+        a.params = c.params;
+        a.out = outs[ci];
+        a.n = c.n;
+        a.P = c.P;
+        a.n_index = c.n_index;
+        a.counter = nullptr;
+        a.tinfo = c.tinfo;
+        a.sched = nullptr;
+        a.wlimit = c.payload_words;
+        const int64_t t = g - c.tile_base;
+        switch (c.dtype) {
+            case GLC_I32: wave_tile<int32_t, OP_PACKED>(a, t, lane); break;
+            case GLC_U32: wave_tile<uint32_t, OP_PACKED>(a, t, lane); break;
+            case GLC_I64: wave_tile<int64_t, OP_PACKED>(a, t, lane); break;
+            default: wave_tile<uint64_t, OP_PACKED>(a, t, lane); break;
+        }
+    }
+}
+
 template <class F> int dispatch_dtype(int dtype, F&& f) {
     switch (dtype) {
         case GLC_I32: return f(int32_t());
@@ -595,6 +633,28 @@ int launch_decode_full(const glc_column* c, void* d_out, int mode, hipStream_t s
         using T = decltype(t);
         return plain ? run<T, OP_PLAIN>(a, sched, s) : run<T, OP_PACKED>(a, sched, s);
     });
+}
+
+int launch_batch_decode(const BatchPlan& b, int wpg, int tpw, hipStream_t s) {
+    if (b.Ttot == 0) return GLC_OK;
+    auto grid = [&](int w, int t) {
+        return dim3(static_cast<unsigned>((b.Ttot + static_cast<int64_t>(w) * t - 1) / (static_cast<int64_t>(w) * t)));
+    };
+#define GLC_BATCH_CASE(W, TP)                                                                                     \
+    if (wpg == W && tpw == TP) {                                                                                  \
+        hipLaunchKernelGGL((k_batch_decode<W, TP>), grid(W, TP), dim3(64 * W), 0, s, b.cols, b.tile_col, b.outs,   \
+                           b.Ttot);                                                                       \
+        GLC_HIP(hipGetLastError());                                                                               \
+        return GLC_OK;                                                                                            \
+    }
+    GLC_BATCH_CASE(4, 1)
+    GLC_BATCH_CASE(2, 1)
+    GLC_BATCH_CASE(8, 1)
+    GLC_BATCH_CASE(1, 1)
+    GLC_BATCH_CASE(4, 2)
+    GLC_BATCH_CASE(2, 4)
+#undef GLC_BATCH_CASE
+    return fail(GLC_ERR_INVALID_ARG, "unsupported batch schedule");
 }
 
 int launch_preunpack(glc_column* c, int64_t* d_plain, hipStream_t s) {

@@ -76,6 +76,65 @@ struct IndexSummary {
     int64_t total_bits;     // max end bit over partitions
 };
 
+// Column table entry of a batch (device resident).  Every array pointer
+// points into the batch's shared allocations.
+struct BatchCol {
+    const void* values;   // compress input (caller's device array)
+    int32_t* start;
+    int32_t* end;
+    int32_t* type;
+    double* params;
+    int32_t* bits;
+    int64_t* off;
+    int64_t* err;
+    Desc* desc;
+    int32_t* tile_part;
+    TileInfo* tinfo;
+    IndexSummary* sum;      // max bits / flags of the column
+    uint32_t* words;
+    int64_t n;
+    int64_t P;
+    int64_t n_index;
+    int64_t payload_words;  // written on the device after the offsets scan
+    int64_t words_reserved; // payload words reserved for the column (worst case)
+    int64_t part_base;      // first global partition of the column
+    int64_t tile_base;      // first global 1024-element tile of the column
+    int32_t dtype;
+    int32_t fit_bs;         // fit reduction width (from the column's partition 0)
+};
+
+// Device-side description of a batch for the batched launches.
+struct BatchPlan {
+    BatchCol* cols;
+    const int32_t* part_col;  // global partition -> column
+    const int32_t* tile_col;  // global 1024-element tile -> column
+    int64_t* col_bits;        // per-column total bits (written by the scan)
+    void* const* outs;        // per-column decompress destinations
+    int64_t Ptot;
+    int64_t Ttot;
+    int32_t C;
+    int32_t S;
+    bool has_dtype[4];
+};
+// One lookup request of a serving step (device resident, built per step).
+struct ServeReq {
+    const Desc* desc;
+    const int32_t* tile_part;
+    const uint32_t* words;
+    const double* params;
+    const int32_t* pos;  // the request's positions (device)
+    void* out;           // the request's results (device)
+    int64_t n;
+    int64_t first;       // index of the request's first lookup in the step
+    int32_t dtype;
+    int32_t fixed;       // fixed partition size, 0 = generic lookup
+};
+int launch_serve_lookup(const ServeReq* d_reqs, int32_t R, int64_t Q, hipStream_t s);
+
+int launch_batch_compress(const BatchPlan& b, hipStream_t s);
+int launch_batch_decode(const BatchPlan& b, int wpg, int tpw, hipStream_t s);
+
+
 }  // namespace glc
 
 // Opaque column handle of the C-ABI.
@@ -109,6 +168,7 @@ struct glc_column {
     uint32_t* d_words = nullptr;    // payload (nullptr after preunpack with free_packed)
     int64_t words_alloc = 0;
     int64_t* d_plain = nullptr;     // pre-unpacked deltas
+    bool view = false;              // arrays belong to a batch; destroy frees nothing
 };
 
 namespace glc {
@@ -125,6 +185,16 @@ int fail(int code, const std::string& msg);
 
 inline int elem_size_of(int dtype) { return (dtype == GLC_I32 || dtype == GLC_U32) ? 4 : 8; }
 inline bool valid_dtype(int dtype) { return dtype >= GLC_I32 && dtype <= GLC_U64; }
+
+// ---- column metadata arena (glc_api.hip) ---------------------------------
+struct ArenaLayout {
+    int64_t start, end, type, bits, params, off, err, desc, tile, tinfo, sched, summary, total;
+};
+ArenaLayout layout_for(int64_t P, int64_t n_index);
+void bind_arena(glc_column* c, char* base, const ArenaLayout& L);
+IndexSummary* summary_ptr(glc_column* c);
+int32_t fit_block_size(int64_t len0);
+inline int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
 
 // ---- launchers implemented in the .hip translation units -----------------
 // compress side (glc_compress.hip)

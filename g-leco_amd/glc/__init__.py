@@ -81,12 +81,27 @@ SIGNATURES = [
     ("glc_direct_random_access_gpu", _int, [_vp, _vp, _i64, _vp, _vp]),
     ("glc_direct_random_access_cpu", _int, [_vp, _vp, _i64, _vp]),
     ("glc_direct_handle_destroy", _int, [_vp]),
-    ("glc_batch_compress", _int, [_int, _vp, _vp, _vp, _i32, _vp, _vp]),
+    ("glc_batch_create", _int, [_int, _vp, _vp, _vp, _i32, _vp]),
+    ("glc_batch_compress", _int, [_vp, _vp]),
     ("glc_batch_decompress", _int, [_vp, _vp, _vp]),
-    ("glc_batch_decompress_graph", _int, [_vp, _vp, _vp]),
+    ("glc_batch_run_graph", _int, [_vp, _int, _vp, _vp]),
     ("glc_batch_set_schedule", _int, [_vp, _i32, _i32]),
+    ("glc_batch_get_schedule", _int, [_vp, _vp, _vp]),
+    ("glc_batch_autotune", _int, [_vp, _vp, _i32, _vp, _vp, _vp, _vp]),
+    ("glc_batch_sizes", _int, [_vp, _vp, _vp]),
     ("glc_batch_get_column", _int, [_vp, _int, _vp]),
     ("glc_batch_num_columns", _int, [_vp]),
+    ("glc_server_create", _int, [_i64, _i64, _i32, _vp]),
+    ("glc_server_add_column", _int, [_vp, _vp, _vp]),
+    ("glc_server_submit_lookup", _int, [_vp, _i32, _vp, _i64, _vp]),
+    ("glc_server_submit_decompress", _int, [_vp, _i32, _vp]),
+    ("glc_server_wait", _int, [_vp, _i64, _i32]),
+    ("glc_server_result", _int, [_vp, _i64, _vp, _vp, _vp]),
+    ("glc_server_copy_result", _int, [_vp, _i64, _vp]),
+    ("glc_server_release", _int, [_vp, _i64]),
+    ("glc_server_stats_get", _int, [_vp, _vp]),
+    ("glc_server_reset_stats", _int, [_vp]),
+    ("glc_server_destroy", _int, [_vp]),
     ("glc_batch_free", _int, [_vp]),
 ]
 
@@ -159,9 +174,9 @@ class Column:
         return self._h
 
     def close(self):
-        if self._h:
+        if self._h and not getattr(self, "_view", False):
             lib().glc_column_free(self._h)
-            self._h = None
+        self._h = None
 
     def __del__(self):
         try:
@@ -338,3 +353,183 @@ class DirectHandle:
             self.close()
         except Exception:
             pass
+
+
+BATCH_COMPRESS = 1
+BATCH_DECOMPRESS = 2
+BATCH_SCHEDULES = [(4, 1), (2, 1), (8, 1), (1, 1), (4, 2), (2, 4)]
+
+
+class Batch:
+    """Many device columns compressed / decompressed by a fixed number of
+    launches (recordable as one hipGraph).  Keeps references to the value
+    tensors, whose memory compress reads."""
+
+    def __init__(self, columns, partition_size: int = 1024):
+        self._h = None
+        self.columns = list(columns)
+        self.partition_size = int(partition_size)
+        C_ = len(self.columns)
+        dts = (C.c_int32 * max(C_, 1))(*[dtype_of(t) for t in self.columns])
+        ptrs = (C.c_void_p * max(C_, 1))(*[t.data_ptr() if t.numel() else 0 for t in self.columns])
+        ns = (C.c_int64 * max(C_, 1))(*[t.numel() for t in self.columns])
+        for t in self.columns:
+            if t.numel():
+                _devptr(t)
+        h = C.c_void_p()
+        _check(lib().glc_batch_create(C_, dts, ptrs, ns, self.partition_size, C.byref(h)))
+        self._h = h
+        self._out_ptrs = None
+
+    def close(self):
+        if self._h:
+            lib().glc_batch_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __len__(self):
+        return len(self.columns)
+
+    def _outs(self, outs):
+        import torch
+        if outs is None:
+            outs = [torch.empty_like(t) for t in self.columns]
+        if len(outs) != len(self.columns):
+            raise ValueError("one output per column")
+        for o, t in zip(outs, self.columns):
+            if o.numel() != t.numel() or o.element_size() != t.element_size() or not o.is_cuda:
+                raise ValueError("output shape/dtype/device mismatch")
+        arr = (C.c_void_p * max(len(outs), 1))(*[o.data_ptr() if o.numel() else 0 for o in outs])
+        return outs, arr
+
+    def compress(self, stream=None):
+        _check(lib().glc_batch_compress(self._h, _stream(stream)))
+
+    def decompress(self, outs=None, stream=None):
+        outs, arr = self._outs(outs)
+        _check(lib().glc_batch_decompress(self._h, arr, _stream(stream)))
+        return outs
+
+    def run_graph(self, what: int, outs=None, stream=None):
+        arr = None
+        if what & BATCH_DECOMPRESS:
+            outs, arr = self._outs(outs)
+        _check(lib().glc_batch_run_graph(self._h, what, arr, _stream(stream)))
+        return outs
+
+    def set_schedule(self, waves_per_group: int, tiles_per_wave: int):
+        _check(lib().glc_batch_set_schedule(self._h, waves_per_group, tiles_per_wave))
+
+    def schedule(self):
+        w, t = C.c_int32(), C.c_int32()
+        _check(lib().glc_batch_get_schedule(self._h, C.byref(w), C.byref(t)))
+        return w.value, t.value
+
+    def autotune(self, outs=None, reps: int = 3, stream=None):
+        outs, arr = self._outs(outs)
+        w, t, ms = C.c_int32(), C.c_int32(), C.c_float()
+        _check(lib().glc_batch_autotune(self._h, arr, reps, _stream(stream), C.byref(w), C.byref(t), C.byref(ms)))
+        return (w.value, t.value), ms.value
+
+    def sizes(self):
+        bits = np.zeros(max(len(self.columns), 1), np.int64)
+        tot = C.c_int64()
+        _check(lib().glc_batch_sizes(self._h, bits.ctypes.data_as(C.c_void_p), C.byref(tot)))
+        return bits[:len(self.columns)], tot.value
+
+    def column(self, i: int) -> "Column":
+        h = C.c_void_p()
+        _check(lib().glc_batch_get_column(self._h, i, C.byref(h)))
+        col = Column(h, dtype_of(self.columns[i]))
+        col._owner = self  # the view lives in the batch's memory and is freed with it
+        col._view = True
+        return col
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("completed", C.c_int64), ("lookups", C.c_int64), ("values_decompressed", C.c_int64),
+                ("steps", C.c_int64), ("mean_requests_per_step", C.c_double), ("p50_ms", C.c_double),
+                ("p99_ms", C.c_double), ("p50_lookup_ms", C.c_double), ("p99_lookup_ms", C.c_double),
+                ("p50_decompress_ms", C.c_double), ("p99_decompress_ms", C.c_double), ("elapsed_s", C.c_double),
+                ("pool_bytes", C.c_int64), ("pool_peak_bytes", C.c_int64), ("inflight_peak", C.c_int32),
+                ("worker_status", C.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Server:
+    """Request loop over resident compressed columns (continuous batching).
+
+    submit_lookup / submit_decompress return request ids; wait(id) blocks
+    until done; result(id) copies the values to a numpy array; release(id)
+    returns the result block to the HBM budget."""
+
+    def __init__(self, hbm_budget_bytes: int, max_inflight: int = 4096, staging_bytes: int = 0):
+        h = C.c_void_p()
+        _check(lib().glc_server_create(int(hbm_budget_bytes), int(staging_bytes), int(max_inflight), C.byref(h)))
+        self._h = h
+        self._cols = []
+
+    def close(self):
+        if self._h:
+            lib().glc_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_column(self, col: "Column") -> int:
+        cid = C.c_int32()
+        _check(lib().glc_server_add_column(self._h, col.handle, C.byref(cid)))
+        self._cols.append(col)
+        return cid.value
+
+    def submit_lookup(self, col_id: int, pos: np.ndarray) -> int:
+        pos = np.ascontiguousarray(pos, dtype=np.int32)
+        rid = C.c_int64()
+        _check(lib().glc_server_submit_lookup(self._h, col_id, pos.ctypes.data_as(C.c_void_p), pos.size,
+                                               C.byref(rid)))
+        return rid.value
+
+    def submit_decompress(self, col_id: int) -> int:
+        rid = C.c_int64()
+        _check(lib().glc_server_submit_decompress(self._h, col_id, C.byref(rid)))
+        return rid.value
+
+    def wait(self, rid: int, timeout_ms: int = -1):
+        _check(lib().glc_server_wait(self._h, rid, timeout_ms))
+
+    def result(self, rid: int, col_id: int) -> np.ndarray:
+        cnt = C.c_int64()
+        ptr = C.c_void_p()
+        lat = C.c_double()
+        _check(lib().glc_server_result(self._h, rid, C.byref(ptr), C.byref(cnt), C.byref(lat)))
+        dt = self._cols[col_id].dtype
+        out = np.empty(cnt.value, {I32: np.int32, U32: np.uint32, I64: np.int64, U64: np.uint64}[dt])
+        _check(lib().glc_server_copy_result(self._h, rid, out.ctypes.data_as(C.c_void_p)))
+        return out
+
+    def latency_ms(self, rid: int) -> float:
+        lat = C.c_double()
+        _check(lib().glc_server_result(self._h, rid, None, None, C.byref(lat)))
+        return lat.value
+
+    def release(self, rid: int):
+        _check(lib().glc_server_release(self._h, rid))
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        _check(lib().glc_server_stats_get(self._h, C.byref(st)))
+        return st.as_dict()
+
+    def reset_stats(self):
+        _check(lib().glc_server_reset_stats(self._h))

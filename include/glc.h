@@ -166,24 +166,94 @@ int glc_direct_random_access_gpu(const glc_direct_handle* h, const int32_t* d_po
 int glc_direct_random_access_cpu(const glc_direct_handle* h, const int32_t* h_pos, int64_t q, void* h_out);
 int glc_direct_handle_destroy(glc_direct_handle* h);
 
-/* ---- batched form (no reference counterpart: many columns, one launch) ---- */
+/* ---- batched form (no reference counterpart: many columns, one launch) ----
+ * A batch holds the compressed form of many device columns of mixed element
+ * types and lengths, each cut into fixed partitions of one size.  Every step
+ * runs as a fixed number of kernel launches whatever the column count, needs
+ * no host synchronisation and no allocation after glc_batch_create, and can
+ * therefore be recorded once into a hipGraph and replayed. */
 typedef struct glc_batch glc_batch;
-/* Compress `num_columns` device columns (values[i], n[i], dtypes[i]) with fixed
- * partitions of `partition_size` in a fixed number of launches independent of
- * the column count. */
-int glc_batch_compress(int num_columns, const int32_t* dtypes, const void* const* d_values, const int64_t* n,
-                       int32_t partition_size, void* stream, glc_batch** out);
-/* Decompress every column of the batch into d_out[i] in one launch. */
-int glc_batch_decompress(const glc_batch* b, void* const* d_out, void* stream);
-/* Capture the batch decompress into a hipGraph and replay it (graph is cached
- * per output-pointer set). */
-int glc_batch_decompress_graph(glc_batch* b, void* const* d_out, void* stream);
-/* Tile/schedule used by the batched kernels: elements per wave tile (256..8192,
- * multiple of 64) and waves per workgroup (1..16).  0 = autotuned/default. */
-int glc_batch_set_schedule(glc_batch* b, int32_t tile_elems, int32_t waves_per_group);
-int glc_batch_get_column(const glc_batch* b, int index, const glc_column** col);
+/* Plan and allocate a batch over device columns values[i] (n[i] elements of
+ * dtypes[i]).  The payload reserve per column is its worst case (33 bits per
+ * 32-bit value, 64 per 64-bit value) so compress never waits for sizes.
+ * The value arrays must stay valid while glc_batch_compress may run. */
+int glc_batch_create(int num_columns, const int32_t* dtypes, const void* const* d_values, const int64_t* n,
+                     int32_t partition_size, glc_batch** out);
+/* Enqueue the compression of every column (fit, offsets, index, pack). */
+int glc_batch_compress(glc_batch* b, void* stream);
+/* Enqueue the full decompression of every column i into d_out[i]. */
+int glc_batch_decompress(glc_batch* b, void* const* d_out, void* stream);
+/* Same work through a hipGraph recorded on first use and replayed after:
+ * what = GLC_BATCH_COMPRESS, GLC_BATCH_DECOMPRESS or both (one graph
+ * launch compresses then decompresses every column).  d_out may be NULL when
+ * what has no decompress. */
+enum { GLC_BATCH_COMPRESS = 1, GLC_BATCH_DECOMPRESS = 2 };
+int glc_batch_run_graph(glc_batch* b, int what, void* const* d_out, void* stream);
+/* Decompress schedule: waves per workgroup x 1024-element tiles per wave.
+ * Supported: (1,1) (2,1) (4,1) (8,1) (4,2) (2,4). */
+int glc_batch_set_schedule(glc_batch* b, int32_t waves_per_group, int32_t tiles_per_wave);
+int glc_batch_get_schedule(const glc_batch* b, int32_t* waves_per_group, int32_t* tiles_per_wave);
+/* Time every supported schedule on this batch with HIP events (decompress
+ * into d_out, `reps` launches each) and keep the fastest.  Synchronises
+ * `stream`.  ms_out (optional) receives the best average launch time. */
+int glc_batch_autotune(glc_batch* b, void* const* d_out, int32_t reps, void* stream, int32_t* waves_per_group,
+                       int32_t* tiles_per_wave, float* ms_out);
+/* Compressed sizes after compress: waits for the last compress, then
+ * per_column_bits[i] (optional) and *total_bytes (optional; bytes of the
+ * serialized layout: 64 B per partition + 120 B header + payload). */
+int glc_batch_sizes(glc_batch* b, int64_t* per_column_bits, int64_t* total_bytes);
+/* Column i as an ordinary column handle (random access, serialize, ...).
+ * The view shares the batch's memory and lives until glc_batch_free; waits
+ * for the last compress. */
+int glc_batch_get_column(glc_batch* b, int index, const glc_column** col);
 int glc_batch_num_columns(const glc_batch* b);
 int glc_batch_free(glc_batch* b);
+
+/* ---- serving form (no reference counterpart) ------------------------------
+ * Compressed columns stay resident in HBM; a worker thread admits lookup and
+ * whole-column decompress requests against a device-memory budget and
+ * batches every lookup admitted together into one kernel launch
+ * (continuous batching: requests arriving while others run join the next
+ * step).  Results stay in the server's device pool until released. */
+typedef struct glc_server glc_server;
+typedef struct {
+    int64_t completed;            /* requests finished since the last reset */
+    int64_t lookups;              /* lookups answered */
+    int64_t values_decompressed;  /* values of whole-column decompress requests */
+    int64_t steps;                /* lookup steps (kernel launches) */
+    double mean_requests_per_step;
+    double p50_ms, p99_ms;        /* submit -> done latency, all requests */
+    double p50_lookup_ms, p99_lookup_ms;
+    double p50_decompress_ms, p99_decompress_ms;
+    double elapsed_s;             /* first submit -> last completion */
+    int64_t pool_bytes;           /* result pool (budget minus staging) */
+    int64_t pool_peak_bytes;
+    int32_t inflight_peak;
+    int32_t worker_status;        /* GLC_OK, or the error that stopped the worker */
+} glc_server_stats;
+/* hbm_budget_bytes: device memory the server may use (lookup staging +
+ * result pool); staging_bytes: pinned/device staging for lookup positions
+ * (0 = 51 MiB, three slots); max_inflight: admitted-but-unreleased cap. */
+int glc_server_create(int64_t hbm_budget_bytes, int64_t staging_bytes, int32_t max_inflight, glc_server** out);
+/* Register a resident column (not owned; must outlive the server). */
+int glc_server_add_column(glc_server* s, const glc_column* col, int32_t* col_id);
+/* Queue q lookups of column col_id (positions copied; out-of-range -> 0).
+ * GLC_ERR_BUDGET when the request can never fit the budget/staging. */
+int glc_server_submit_lookup(glc_server* s, int32_t col_id, const int32_t* h_pos, int64_t q, int64_t* req_id);
+/* Queue a full decompression of column col_id. */
+int glc_server_submit_decompress(glc_server* s, int32_t col_id, int64_t* req_id);
+/* Wait for a request (timeout_ms < 0: no limit; GLC_ERR_STATE on timeout). */
+int glc_server_wait(glc_server* s, int64_t req_id, int32_t timeout_ms);
+/* Device address of a finished request's results (valid until release). */
+int glc_server_result(glc_server* s, int64_t req_id, const void** d_out, int64_t* count, double* latency_ms);
+/* Copy a finished request's results to host memory (count * element bytes). */
+int glc_server_copy_result(glc_server* s, int64_t req_id, void* h_dst);
+/* Return a finished request's result block to the pool. */
+int glc_server_release(glc_server* s, int64_t req_id);
+int glc_server_stats_get(glc_server* s, glc_server_stats* st);
+int glc_server_reset_stats(glc_server* s);
+/* Finish in-flight work, stop the worker and free the server. */
+int glc_server_destroy(glc_server* s);
 
 #ifdef __cplusplus
 }

@@ -100,3 +100,14 @@ def test_direct_handle_rejects_out_of_range_bit_ranges():
     with pytest.raises(glc.GlcError) as e:
         glc.DirectHandle(bytes(blob), glc.I32, copy_to_device=False)
     assert e.value.code == -4
+
+
+def test_batch_and_server_argument_checks_need_no_device():
+    L = glc.lib()
+    h = C.c_void_p()
+    assert L.glc_batch_create(-1, None, None, None, 1024, C.byref(h)) == -1
+    assert L.glc_batch_create(1, None, None, None, 1024, C.byref(h)) == -1
+    assert L.glc_server_create(1 << 30, 0, 0, C.byref(h)) == -1      # max_inflight < 1
+    assert L.glc_server_create(1 << 20, 0, 16, C.byref(h)) == -1     # budget below the staging
+    assert L.glc_server_create(1 << 30, 16, 16, C.byref(h)) == -1    # staging too small
+    assert not h.value
