@@ -403,6 +403,20 @@ int glc_column_free(glc_column* col) {
     return GLC_OK;
 }
 
+int glc_column_arrays_get(const glc_column* c, glc_column_arrays* a) {
+    if (!c || !a) return fail(GLC_ERR_INVALID_ARG, "NULL argument");
+    a->start_indices = c->d_start;
+    a->end_indices = c->d_end;
+    a->model_types = c->d_type;
+    a->model_params = c->d_params;
+    a->delta_bits = c->d_bits;
+    a->bit_offsets = c->d_off;
+    a->error_bounds = c->d_err;
+    a->delta_array = c->d_words;
+    a->plain_deltas = c->d_plain;
+    return GLC_OK;
+}
+
 int glc_column_info_get(const glc_column* c, glc_column_info* info) {
     if (!c || !info) return fail(GLC_ERR_INVALID_ARG, "NULL argument");
     info->dtype = c->dtype;

@@ -347,41 +347,49 @@ int admit_and_launch(glc_server* s) {
 
 void worker_main(glc_server* s) {
     if (hipSetDevice(s->device) != hipSuccess) {
+        std::lock_guard<std::mutex> lk(s->mu);
         s->worker_rc = GLC_ERR_HIP;
         s->worker_err = "hipSetDevice failed in the serving worker";
-        return;
     }
     for (;;) {
+        bool stopping;
         {
             std::unique_lock<std::mutex> lk(s->mu);
-            const bool busy = s->inflight > 0;
-            if (!busy && s->queue.empty() && s->to_free.empty()) {
+            stopping = s->stop || s->worker_rc != GLC_OK;
+            if (stopping && s->inflight == 0) {
+                // fail what was never admitted, then leave
+                while (!s->queue.empty()) {
+                    s->pending.push_back(s->queue.front());
+                    s->queue.pop_front();
+                }
+                const int rc = s->worker_rc != GLC_OK ? s->worker_rc : GLC_ERR_STATE;
+                for (Req* r : s->pending) complete(s, r, rc, now_s());
+                s->pending.clear();
+                s->cv_done.notify_all();
                 if (s->stop) break;
                 s->cv_work.wait_for(lk, std::chrono::milliseconds(50));
                 continue;
             }
-            if (s->stop && s->queue.empty() && s->pending.empty() && !busy) break;
+            if (!stopping && s->inflight == 0 && s->queue.empty() && s->to_free.empty()) {
+                // idle, or waiting for a client to release results
+                s->cv_work.wait_for(lk, std::chrono::milliseconds(s->pending.empty() ? 50 : 1));
+                if (s->queue.empty() && s->to_free.empty()) continue;
+            }
         }
         retire(s);
-        const int rc = admit_and_launch(s);
-        if (rc) {
-            std::lock_guard<std::mutex> lk(s->mu);
-            s->worker_rc = rc;
-            s->worker_err = glc_last_error();
-            // fail everything not yet launched
-            for (Req* r : s->pending) complete(s, r, rc, now_s());
-            s->pending.clear();
-            s->cv_done.notify_all();
+        if (!stopping) {
+            const int rc = admit_and_launch(s);
+            if (rc) {
+                std::lock_guard<std::mutex> lk(s->mu);
+                s->worker_rc = rc;
+                s->worker_err = glc_last_error();
+            }
         }
         {
             std::unique_lock<std::mutex> lk(s->mu);
-            const bool busy = s->inflight > 0;
-            if (busy && s->queue.empty()) {
+            if (s->inflight > 0 && s->queue.empty()) {
                 lk.unlock();
                 std::this_thread::sleep_for(std::chrono::microseconds(5));
-            } else if (!busy && s->queue.empty() && !s->pending.empty()) {
-                // waiting for clients to release results
-                s->cv_work.wait_for(lk, std::chrono::milliseconds(1));
             }
         }
     }

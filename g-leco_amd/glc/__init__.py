@@ -81,6 +81,7 @@ SIGNATURES = [
     ("glc_direct_random_access_gpu", _int, [_vp, _vp, _i64, _vp, _vp]),
     ("glc_direct_random_access_cpu", _int, [_vp, _vp, _i64, _vp]),
     ("glc_direct_handle_destroy", _int, [_vp]),
+    ("glc_column_arrays_get", _int, [_vp, _vp]),
     ("glc_batch_create", _int, [_int, _vp, _vp, _vp, _i32, _vp]),
     ("glc_batch_compress", _int, [_vp, _vp]),
     ("glc_batch_decompress", _int, [_vp, _vp, _vp]),

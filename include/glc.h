@@ -112,6 +112,20 @@ int glc_column_info_get(const glc_column* col, glc_column_info* info);
 /* Copy the SoA metadata to host arrays (any may be NULL). params is [4*P]. */
 int glc_column_copy_meta(const glc_column* col, int32_t* start, int32_t* end, int32_t* model_type, double* params,
                          int32_t* delta_bits, int64_t* bit_offsets, int64_t* error_bounds);
+/* Device SoA arrays of CompressedData<T> (api/G-LeCo_Types.cuh:18-42);
+ * words / plain_deltas are NULL when absent.  Owned by the column. */
+typedef struct {
+    int32_t* start_indices;
+    int32_t* end_indices;
+    int32_t* model_types;
+    double* model_params;
+    int32_t* delta_bits;
+    int64_t* bit_offsets;
+    int64_t* error_bounds;
+    uint32_t* delta_array;
+    int64_t* plain_deltas;
+} glc_column_arrays;
+int glc_column_arrays_get(const glc_column* col, glc_column_arrays* out);
 /* Copy the bit-packed payload (ceil(total_bits/32) words) to the host. */
 int glc_column_copy_payload(const glc_column* col, uint32_t* h_words, int64_t n_words);
 /* Partition boundaries the chosen partitioner would produce, without fitting
