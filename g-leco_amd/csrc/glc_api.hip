@@ -221,8 +221,16 @@ int compress_impl(int dtype, const void* d_values, int64_t n, int partitioner, i
             len0 = static_cast<int64_t>(first[1]) - first[0];
             break;
         }
-        case GLC_PART_GPU_SPLITMERGE:
-            return bail(fail(GLC_ERR_UNSUPPORTED, "GPU split/merge partitioner is not available in this build"));
+        case GLC_PART_GPU_SPLITMERGE: {
+            std::vector<int32_t> st, en;
+            if ((rc = gpu_split_merge(dtype, d_values, n, min_size, st, en, s))) return bail(rc);
+            if ((rc = alloc_column(c, static_cast<int64_t>(st.size()), s))) return bail(rc);
+            GLC_HIP(hipMemcpyAsync(c->d_start, st.data(), st.size() * 4, hipMemcpyHostToDevice, s));
+            GLC_HIP(hipMemcpyAsync(c->d_end, en.data(), en.size() * 4, hipMemcpyHostToDevice, s));
+            GLC_HIP(hipStreamSynchronize(s));
+            len0 = en[0] - st[0];
+            break;
+        }
         case GLC_PART_CPU_SPLITMERGE: {
             std::vector<char> h(static_cast<size_t>(n) * c->elem_size);
             GLC_HIP(hipMemcpyAsync(h.data(), d_values, h.size(), hipMemcpyDeviceToHost, s));

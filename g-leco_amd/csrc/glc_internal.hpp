@@ -227,6 +227,9 @@ struct BlobView {            // device pointers into a serialized blob
 };
 int launch_direct_access(int dtype, const BlobView& v, const int32_t* d_pos, int64_t q, void* d_out, hipStream_t s);
 
+// GPU split/merge partitioner (glc_splitmerge.hip); boundaries returned on the host
+int gpu_split_merge(int dtype, const void* d_values, int64_t n, int32_t min_size, std::vector<int32_t>& st,
+                    std::vector<int32_t>& en, hipStream_t s);
 // host-side CPU partitioner (glc_host.cpp)
 int cpu_split_merge(int dtype, const void* h_values, int64_t n, int32_t min_size, std::vector<int32_t>& st,
                     std::vector<int32_t>& en);

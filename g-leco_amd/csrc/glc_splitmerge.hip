@@ -1,0 +1,513 @@
+// glc_splitmerge.hip -- GPU split/merge partitioner (GLC_PART_GPU_SPLITMERGE).
+//
+// Algorithm of the reference's VariableLengthPartitioner
+// (compression/CpuPartitioner.cuh:141-242) with its cost model (:78-138:
+// 80 B + n*bits/8, linear fit without FMA, the int32 n*(n-1) of :56-72), the
+// normal-equation sums Sy, Sxy taken exactly in 128-bit integers and rounded
+// once to double.  Exact sums are independent of summation order, so the
+// parallel reductions here give the same costs -- hence the same boundaries --
+// as a sequential evaluation of the same definition.
+//
+//   split   every start s = j*m (m = minimum size) is evaluated at once, one
+//           workgroup each: the candidate sizes m*{1,2,3,4,5,7,...,17,21,25,29}
+//           (step m, then 2m above 4m, then 4m above 16m, at most 32m) share
+//           one pass for the sums (nested prefixes) and one for the maximum
+//           error; the smallest cost per value wins (first on ties);
+//   chain   the greedy walk 0 -> end(0) -> end(end(0)) ... keeps the starts it
+//           reaches: pointer doubling marks them in log2(N/m) rounds;
+//   merge   up to 10 left-to-right passes.  Pass: the merged cost of every
+//           adjacent pair in parallel, then one workgroup walks the list;
+//           while a group is a single partition its decision is the
+//           precomputed pair test, otherwise the workgroup fits the grown
+//           group on the spot (the reference's sequential semantics).
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <vector>
+
+#include "glc_device.hpp"
+#include "glc_internal.hpp"
+
+namespace glc {
+namespace {
+
+constexpr int kSmThreads = 256;
+constexpr int kMaxCand = 14;  // m * {1,2,3,4,5,7,9,11,13,15,17,21,25,29}
+__constant__ int kCandUnits[kMaxCand] = {1, 2, 3, 4, 5, 7, 9, 11, 13, 15, 17, 21, 25, 29};
+
+typedef __int128 i128;
+typedef unsigned __int128 u128;
+
+template <class T> __device__ __forceinline__ i128 wide(T x) {
+    if constexpr (std::is_signed<T>::value) return static_cast<i128>(static_cast<int64_t>(x));
+    else return static_cast<i128>(static_cast<uint64_t>(x));
+}
+
+// Round-to-nearest-even conversion of a signed 128-bit integer to double.
+__device__ double i128_to_double(i128 x) {
+    const bool neg = x < 0;
+    const u128 m = neg ? static_cast<u128>(0) - static_cast<u128>(x) : static_cast<u128>(x);
+    const uint64_t hi = static_cast<uint64_t>(m >> 64), lo = static_cast<uint64_t>(m);
+    if (hi == 0 && lo == 0) return 0.0;
+    const int L = hi ? 127 - __builtin_clzll(hi) : 63 - __builtin_clzll(lo);
+    double r;
+    if (L <= 52) {
+        r = static_cast<double>(lo);
+    } else {
+        const int sh = L - 52;
+        uint64_t mant = static_cast<uint64_t>(m >> sh);
+        const u128 rem = m & ((static_cast<u128>(1) << sh) - 1);
+        const u128 half = static_cast<u128>(1) << (sh - 1);
+        if (rem > half || (rem == half && (mant & 1))) ++mant;
+        r = __builtin_ldexp(static_cast<double>(mant), sh);  // mant <= 2^53: exact
+    }
+    return neg ? -r : r;
+}
+
+__device__ __forceinline__ i128 shfl_xor_i128(i128 v, int o) {
+    const uint64_t lo = static_cast<uint64_t>(static_cast<u128>(v));
+    const uint64_t hi = static_cast<uint64_t>(static_cast<u128>(v) >> 64);
+    const uint64_t l2 = __shfl_xor(lo, o, 64), h2 = __shfl_xor(hi, o, 64);
+    return static_cast<i128>((static_cast<u128>(h2) << 64) | l2);
+}
+__device__ __forceinline__ i128 wave_sum(i128 v) {
+    for (int o = 32; o > 0; o >>= 1) v += shfl_xor_i128(v, o);
+    return v;
+}
+__device__ __forceinline__ int64_t wave_max(int64_t v) {
+    for (int o = 32; o > 0; o >>= 1) {
+        const int64_t y = __shfl_xor(v, o, 64);
+        v = y > v ? y : v;
+    }
+    return v;
+}
+
+// theta of the split/merge cost model from exact sums over n values.
+__device__ void sm_theta(i128 Sy, i128 Sxy, int64_t n, double& t0, double& t1) {
+    const i128 N = n;
+    const i128 Sxx = (N - 1) * N * (2 * N - 1) / 6;
+    const int32_t nn1 = static_cast<int32_t>(static_cast<uint32_t>(n) * static_cast<uint32_t>(n - 1));
+    const double sx = static_cast<double>(nn1) / 2.0;
+    const double sy = i128_to_double(Sy), sxy = i128_to_double(Sxy), sxx = i128_to_double(Sxx);
+    const double dn = static_cast<double>(n);
+    const double det = dn * sxx - sx * sx;
+    if (__builtin_fabs(det) > 1e-10) {
+        t1 = (dn * sxy - sx * sy) / det;
+        t0 = (sy - t1 * sx) / dn;
+    } else {
+        t1 = 0.0;
+        t0 = sy / dn;
+    }
+}
+
+template <class T> __device__ __forceinline__ int64_t abs_err(T v, double t0, double t1, int64_t i) {
+    const double pr = t0 + t1 * static_cast<double>(i);  // no contraction (-ffp-contract=off)
+    const int64_t d = calc_delta<T>(v, sat_cast<T>(__builtin_round(pr)));
+    return d < 0 ? static_cast<int64_t>(0ULL - static_cast<uint64_t>(d)) : d;
+}
+
+__device__ __forceinline__ int32_t bits_of(int64_t mx) {
+    if (mx <= 0) return 0;
+    const int b = 64 - __builtin_clzll(static_cast<unsigned long long>(mx)) + 1;
+    return b > 64 ? 64 : b;
+}
+
+__device__ __forceinline__ double cost_of(int64_t n, int32_t bits) {
+    return 80.0 + static_cast<double>(n) * bits / 8.0;  // CpuPartitioner.cuh:128-138
+}
+
+struct FitLds {
+    i128 sy[kSmThreads / 64];
+    i128 sxy[kSmThreads / 64];
+    int64_t mx[kSmThreads / 64];
+    double t0, t1, cost;
+};
+
+// Cost of [s, e) computed by the whole workgroup (all threads call it).
+template <class T> __device__ double wg_cost(const T* __restrict__ v, int64_t s, int64_t e, FitLds& L) {
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int64_t n = e - s;
+    i128 sy = 0, sxy = 0;
+    for (int64_t i = tid; i < n; i += kSmThreads) {
+        const i128 y = wide<T>(v[s + i]);
+        sy += y;
+        sxy += y * static_cast<i128>(i);
+    }
+    sy = wave_sum(sy);
+    sxy = wave_sum(sxy);
+    if (lane == 0) {
+        L.sy[w] = sy;
+        L.sxy[w] = sxy;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        i128 a = 0, b = 0;
+        for (int k = 0; k < kSmThreads / 64; ++k) {
+            a += L.sy[k];
+            b += L.sxy[k];
+        }
+        sm_theta(a, b, n, L.t0, L.t1);
+    }
+    __syncthreads();
+    const double t0 = L.t0, t1 = L.t1;
+    int64_t mx = 0;
+    for (int64_t i = tid; i < n; i += kSmThreads) {
+        const int64_t a = abs_err<T>(v[s + i], t0, t1, i);
+        mx = a > mx ? a : mx;
+    }
+    mx = wave_max(mx);
+    if (lane == 0) L.mx[w] = mx;
+    __syncthreads();
+    if (tid == 0) {
+        int64_t m = L.mx[0];
+        for (int k = 1; k < kSmThreads / 64; ++k) m = L.mx[k] > m ? L.mx[k] : m;
+        L.cost = cost_of(n, bits_of(m));
+    }
+    __syncthreads();
+    const double c = L.cost;
+    __syncthreads();  // L is reused by the caller's next fit
+    return c;
+}
+
+// ---- split: one workgroup per start j*m ------------------------------------
+template <class T>
+__global__ __launch_bounds__(kSmThreads) void k_sm_split(const T* __restrict__ v, int64_t n, int32_t m, int64_t J,
+                                                         int32_t* __restrict__ best_end,
+                                                         double* __restrict__ best_cost) {
+    __shared__ i128 s_seg[kMaxCand][2];
+    __shared__ double s_t[kMaxCand][2];
+    __shared__ int64_t s_mx[kSmThreads / 64][kMaxCand];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int64_t j = blockIdx.x;
+    if (j >= J) return;
+    const int64_t cur = j * m;
+    const int64_t max_size = (n - cur < static_cast<int64_t>(m) * 32) ? n - cur : static_cast<int64_t>(m) * 32;
+    int K = 0;
+    while (K < kMaxCand && static_cast<int64_t>(kCandUnits[K]) * m <= max_size) ++K;
+    if (K == 0) {  // fewer than m values left: one partition [cur, n)
+        __shared__ FitLds L;
+        const double c = wg_cost<T>(v, cur, n, L);
+        if (tid == 0) {
+            best_end[j] = static_cast<int32_t>(n);
+            best_cost[j] = c;
+        }
+        return;
+    }
+    const int64_t Lmax = static_cast<int64_t>(kCandUnits[K - 1]) * m;
+    // segment sums: segment k = [size(k-1), size(k)), one wave each
+    for (int k = w; k < K; k += kSmThreads / 64) {
+        const int64_t a = (k == 0) ? 0 : static_cast<int64_t>(kCandUnits[k - 1]) * m;
+        const int64_t b = static_cast<int64_t>(kCandUnits[k]) * m;
+        i128 sy = 0, sxy = 0;
+        for (int64_t i = a + lane; i < b; i += 64) {
+            const i128 y = wide<T>(v[cur + i]);
+            sy += y;
+            sxy += y * static_cast<i128>(i);
+        }
+        sy = wave_sum(sy);
+        sxy = wave_sum(sxy);
+        if (lane == 0) {
+            s_seg[k][0] = sy;
+            s_seg[k][1] = sxy;
+        }
+    }
+    __syncthreads();
+    if (tid < K) {
+        i128 sy = 0, sxy = 0;
+        for (int k = 0; k <= tid; ++k) {
+            sy += s_seg[k][0];
+            sxy += s_seg[k][1];
+        }
+        double t0, t1;
+        sm_theta(sy, sxy, static_cast<int64_t>(kCandUnits[tid]) * m, t0, t1);
+        s_t[tid][0] = t0;
+        s_t[tid][1] = t1;
+    }
+    __syncthreads();
+    int64_t mx[kMaxCand];
+#pragma unroll
+    for (int k = 0; k < kMaxCand; ++k) mx[k] = 0;
+    for (int64_t i = tid; i < Lmax; i += kSmThreads) {
+        const T x = v[cur + i];
+#pragma unroll
+        for (int k = 0; k < kMaxCand; ++k) {
+            if (k < K && i < static_cast<int64_t>(kCandUnits[k]) * m) {
+                const int64_t a = abs_err<T>(x, s_t[k][0], s_t[k][1], i);
+                mx[k] = a > mx[k] ? a : mx[k];
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < kMaxCand; ++k) {
+        const int64_t r = wave_max(mx[k]);
+        if (lane == 0) s_mx[w][k] = r;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double best = DBL_MAX, bc = 0.0;
+        int64_t be = (cur + m < n) ? cur + m : n;
+        for (int k = 0; k < K; ++k) {
+            int64_t r = s_mx[0][k];
+            for (int q = 1; q < kSmThreads / 64; ++q) r = s_mx[q][k] > r ? s_mx[q][k] : r;
+            const int64_t size = static_cast<int64_t>(kCandUnits[k]) * m;
+            const double c = cost_of(size, bits_of(r));
+            const double cpe = c / static_cast<double>(size);
+            if (cpe < best) {
+                best = cpe;
+                be = cur + size;
+                bc = c;
+            }
+        }
+        best_end[j] = static_cast<int32_t>(be);
+        best_cost[j] = bc;
+    }
+}
+
+// ---- chain: mark the starts the greedy walk from 0 reaches -----------------
+__global__ void k_sm_chain_init(const int32_t* __restrict__ best_end, int64_t J, int32_t m,
+                                int32_t* __restrict__ jump, int32_t* __restrict__ mark) {
+    const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j >= J) return;
+    const int64_t e = best_end[j];
+    jump[j] = static_cast<int32_t>((e % m == 0 && e / m < J) ? e / m : J);  // J = past the end
+    mark[j] = (j == 0);
+}
+// marked nodes mark their jump target; then jump <- jump o jump
+__global__ void k_sm_chain_mark(const int32_t* __restrict__ jump, int64_t J, int32_t* __restrict__ mark) {
+    const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j >= J || !mark[j]) return;
+    const int32_t t = jump[j];
+    if (t < J) mark[t] = 1;
+}
+__global__ void k_sm_chain_double(const int32_t* __restrict__ jin, int64_t J, int32_t* __restrict__ jout) {
+    const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j >= J) return;
+    const int32_t t = jin[j];
+    jout[j] = (t < J) ? jin[t] : static_cast<int32_t>(J);
+}
+
+// exclusive scan of mark[] into pos[]; *count = total (one workgroup)
+constexpr int kScan = 1024;
+__global__ __launch_bounds__(kScan) void k_sm_scan(const int32_t* __restrict__ mark, int64_t J,
+                                                   int32_t* __restrict__ pos, int64_t* __restrict__ count) {
+    __shared__ int64_t s[kScan];
+    const int tid = threadIdx.x;
+    const int64_t chunk = (J + kScan - 1) / kScan;
+    const int64_t b0 = tid * chunk;
+    const int64_t b1 = (b0 + chunk < J) ? b0 + chunk : J;
+    int64_t local = 0;
+    for (int64_t i = b0; i < b1; ++i) local += mark[i];
+    s[tid] = local;
+    __syncthreads();
+    for (int o = 1; o < kScan; o <<= 1) {
+        const int64_t y = (tid >= o) ? s[tid - o] : 0;
+        __syncthreads();
+        s[tid] += y;
+        __syncthreads();
+    }
+    int64_t acc = s[tid] - local;
+    for (int64_t i = b0; i < b1; ++i) {
+        pos[i] = static_cast<int32_t>(acc);
+        acc += mark[i];
+    }
+    if (tid == kScan - 1) *count = s[tid];
+}
+__global__ void k_sm_emit(const int32_t* __restrict__ mark, const int32_t* __restrict__ pos,
+                          const int32_t* __restrict__ best_end, const double* __restrict__ best_cost, int64_t J,
+                          int32_t m, int32_t* __restrict__ st, int32_t* __restrict__ en, double* __restrict__ cost) {
+    const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j >= J || !mark[j]) return;
+    const int32_t p = pos[j];
+    st[p] = static_cast<int32_t>(j * m);
+    en[p] = best_end[j];
+    cost[p] = best_cost[j];
+}
+
+// ---- merge -------------------------------------------------------------------
+// pair test of partitions i, i+1 as a fresh group: merged cost and decision
+template <class T>
+__global__ __launch_bounds__(kSmThreads) void k_sm_pair(const T* __restrict__ v, const int32_t* __restrict__ st,
+                                                        const int32_t* __restrict__ en,
+                                                        const double* __restrict__ cost, int64_t P,
+                                                        double* __restrict__ mc, int32_t* __restrict__ flag) {
+    __shared__ FitLds L;
+    for (int64_t i = blockIdx.x; i + 1 < P; i += gridDim.x) {
+        const double c = wg_cost<T>(v, st[i], en[i + 1], L);
+        if (threadIdx.x == 0) {
+            mc[i] = c;
+            flag[i] = c < cost[i] + cost[i + 1];
+        }
+    }
+}
+
+// the sequential pass: out lists + [count, changed]
+constexpr int kWin = 256;
+template <class T>
+__global__ __launch_bounds__(kSmThreads) void k_sm_walk(const T* __restrict__ v, const int32_t* __restrict__ st,
+                                                        const int32_t* __restrict__ en,
+                                                        const double* __restrict__ cost,
+                                                        const double* __restrict__ mc,
+                                                        const int32_t* __restrict__ flag, int64_t P,
+                                                        int32_t* __restrict__ ost, int32_t* __restrict__ oen,
+                                                        double* __restrict__ ocost, int64_t* __restrict__ result) {
+    __shared__ FitLds L;
+    __shared__ int32_t w_st[kWin + 1], w_en[kWin + 1], w_flag[kWin + 1];
+    __shared__ double w_c[kWin + 1], w_mc[kWin + 1];
+    const int tid = threadIdx.x;
+    int64_t base = -1;  // window holds entries [base, base + kWin]
+    auto load = [&](int64_t b) {
+        __syncthreads();
+        for (int k = tid; k <= kWin; k += kSmThreads) {
+            const int64_t i = b + k;
+            if (i < P) {
+                w_st[k] = st[i];
+                w_en[k] = en[i];
+                w_c[k] = cost[i];
+                w_mc[k] = (i + 1 < P) ? mc[i] : 0.0;
+                w_flag[k] = (i + 1 < P) ? flag[i] : 0;
+            }
+        }
+        __syncthreads();
+    };
+    load(0);
+    base = 0;
+    int64_t gs = w_st[0], ge = w_en[0];
+    double gc = w_c[0];
+    bool single = true;
+    int64_t out = 0;
+    int64_t changed = 0;
+    for (int64_t i = 1; i < P; ++i) {
+        if (i - base >= kWin) {  // entries i-1 and i must be in the window
+            base = i - 1;
+            load(base);
+        }
+        const int k = static_cast<int>(i - base);
+        const int32_t si = w_st[k], ei = w_en[k];
+        const double ci = w_c[k];
+        bool merge;
+        double merged;
+        if (single) {
+            merge = w_flag[k - 1] != 0;
+            merged = w_mc[k - 1];
+        } else {
+            merged = wg_cost<T>(v, gs, ei, L);
+            merge = merged < gc + ci;
+        }
+        if (merge) {
+            ge = ei;
+            gc = merged;
+            single = false;
+            changed = 1;
+        } else {
+            if (tid == 0) {
+                ost[out] = static_cast<int32_t>(gs);
+                oen[out] = static_cast<int32_t>(ge);
+                ocost[out] = gc;
+            }
+            ++out;
+            gs = si;
+            ge = ei;
+            gc = ci;
+            single = true;
+        }
+    }
+    if (tid == 0) {
+        ost[out] = static_cast<int32_t>(gs);
+        oen[out] = static_cast<int32_t>(ge);
+        ocost[out] = gc;
+        result[0] = out + 1;
+        result[1] = changed;
+    }
+}
+
+template <class F> int dispatch_dtype(int dtype, F&& f) {
+    switch (dtype) {
+        case GLC_I32: return f(int32_t());
+        case GLC_U32: return f(uint32_t());
+        case GLC_I64: return f(int64_t());
+        case GLC_U64: return f(uint64_t());
+        default: return fail(GLC_ERR_INVALID_ARG, "bad dtype");
+    }
+}
+
+unsigned blocks(int64_t n, int b) { return static_cast<unsigned>((n + b - 1) / b); }
+
+}  // namespace
+
+// Partition d_values[0, n) on the device; the boundaries end up in host
+// vectors (the caller uploads them into the column arena).
+int gpu_split_merge(int dtype, const void* d_values, int64_t n, int32_t m, std::vector<int32_t>& st,
+                    std::vector<int32_t>& en, hipStream_t s) {
+    st.clear();
+    en.clear();
+    if (n <= 0) return GLC_OK;
+    if (m <= 0 || m > (1 << 20)) return fail(GLC_ERR_INVALID_ARG, "split/merge minimum size must be in [1, 2^20]");
+    const int64_t J = (n + m - 1) / m;
+    // scratch: best_end, jump x2, mark, pos, st x2, en x2, flag (int32 x J each);
+    // best_cost, cost x2, mc (double x J each); result (2 x int64)
+    const int64_t i4 = align_up(J * 4, 256), d8 = align_up(J * 8, 256);
+    void* mem = nullptr;
+    if (hipMalloc(&mem, static_cast<size_t>(10 * i4 + 4 * d8 + 256)) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(GLC_ERR_OOM, "split/merge scratch allocation failed");
+    }
+    char* p = static_cast<char*>(mem);
+    auto I = [&](int k) { return reinterpret_cast<int32_t*>(p + k * i4); };
+    auto D = [&](int k) { return reinterpret_cast<double*>(p + 10 * i4 + k * d8); };
+    int32_t *best_end = I(0), *jump0 = I(1), *jump1 = I(2), *mark = I(3), *pos = I(4);
+    int32_t *st0 = I(5), *st1 = I(6), *en0 = I(7), *en1 = I(8), *flag = I(9);
+    double *best_cost = D(0), *cost0 = D(1), *cost1 = D(2), *mc = D(3);
+    int64_t* result = reinterpret_cast<int64_t*>(p + 10 * i4 + 4 * d8);
+    int rc = dispatch_dtype(dtype, [&](auto t) -> int {
+        using T = decltype(t);
+        const T* v = static_cast<const T*>(d_values);
+        hipLaunchKernelGGL(k_sm_split<T>, dim3(static_cast<unsigned>(J)), dim3(kSmThreads), 0, s, v, n, m, J,
+                           best_end, best_cost);
+        GLC_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_sm_chain_init, dim3(blocks(J, 256)), dim3(256), 0, s, best_end, J, m, jump0, mark);
+        GLC_HIP(hipGetLastError());
+        int32_t *ja = jump0, *jb = jump1;
+        for (int64_t reach = 1; reach < J + 1; reach <<= 1) {
+            hipLaunchKernelGGL(k_sm_chain_mark, dim3(blocks(J, 256)), dim3(256), 0, s, ja, J, mark);
+            hipLaunchKernelGGL(k_sm_chain_double, dim3(blocks(J, 256)), dim3(256), 0, s, ja, J, jb);
+            GLC_HIP(hipGetLastError());
+            std::swap(ja, jb);
+        }
+        hipLaunchKernelGGL(k_sm_scan, dim3(1), dim3(kScan), 0, s, mark, J, pos, result);
+        hipLaunchKernelGGL(k_sm_emit, dim3(blocks(J, 256)), dim3(256), 0, s, mark, pos, best_end, best_cost, J, m,
+                           st0, en0, cost0);
+        GLC_HIP(hipGetLastError());
+        int64_t P = 0;
+        GLC_HIP(hipMemcpyAsync(&P, result, 8, hipMemcpyDeviceToHost, s));
+        GLC_HIP(hipStreamSynchronize(s));
+        if (P < 1 || P > J) return fail(GLC_ERR_STATE, "split phase produced an invalid partition count");
+        int32_t *sa = st0, *sb = st1, *ea = en0, *eb = en1;
+        double *ca = cost0, *cb = cost1;
+        for (int iter = 0; iter < 10 && P > 1; ++iter) {  // CpuPartitioner.cuh:198-242
+            const unsigned g = static_cast<unsigned>(std::min<int64_t>(P - 1, 1 << 20));
+            hipLaunchKernelGGL(k_sm_pair<T>, dim3(g), dim3(kSmThreads), 0, s, v, sa, ea, ca, P, mc, flag);
+            hipLaunchKernelGGL(k_sm_walk<T>, dim3(1), dim3(kSmThreads), 0, s, v, sa, ea, ca, mc, flag, P, sb, eb, cb,
+                               result);
+            GLC_HIP(hipGetLastError());
+            int64_t r[2] = {0, 0};
+            GLC_HIP(hipMemcpyAsync(r, result, 16, hipMemcpyDeviceToHost, s));
+            GLC_HIP(hipStreamSynchronize(s));
+            if (r[0] < 1 || r[0] > P) return fail(GLC_ERR_STATE, "merge pass produced an invalid partition count");
+            P = r[0];
+            std::swap(sa, sb);
+            std::swap(ea, eb);
+            std::swap(ca, cb);
+            if (!r[1]) break;
+        }
+        st.resize(static_cast<size_t>(P));
+        en.resize(static_cast<size_t>(P));
+        GLC_HIP(hipMemcpyAsync(st.data(), sa, P * 4, hipMemcpyDeviceToHost, s));
+        GLC_HIP(hipMemcpyAsync(en.data(), ea, P * 4, hipMemcpyDeviceToHost, s));
+        GLC_HIP(hipStreamSynchronize(s));
+        return GLC_OK;
+    });
+    (void)hipFree(mem);
+    return rc;
+}
+
+}  // namespace glc

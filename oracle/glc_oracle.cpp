@@ -641,6 +641,166 @@ template <class T> struct CpuPartitioner {
     }
 };
 
+// Split/merge partitioner with exact normal-equation sums (the build's GPU
+// split/merge, GLC_PART_GPU_SPLITMERGE).  Algorithm of
+// compression/CpuPartitioner.cuh:141-242 -- greedy split over the candidate
+// sizes m*{1,2,3,4,5,7,...,17,21,25,29}, then up to 10 left-to-right merge
+// passes -- with the cost of compression/CpuPartitioner.cuh:78-138 (80 B +
+// n*bits/8, fit without FMA, the int32 n*(n-1) of :56-72) except that Sy and
+// Sxy are exact integer sums (rounded once to double) instead of differences
+// of a double prefix sum over the whole column.  Where the reference's double
+// sums are exact (prefix sums of |v| below 2^53, partitions below 2^18
+// values) both give the same costs, hence the same boundaries; this is
+// checked against CpuPartitioner above on such data.  Exact sums make the
+// result independent of summation order, so the GPU's parallel reductions
+// reproduce it bit for bit.
+struct I128 {
+    __int128 v = 0;
+};
+// Round-to-nearest-even conversion of a signed 128-bit integer to double.
+// Pinned by tests/test_oracle_splitmerge.py against Python's int -> float.
+inline double i128_to_double(__int128 x) {
+    const bool neg = x < 0;
+    unsigned __int128 m = neg ? static_cast<unsigned __int128>(0) - static_cast<unsigned __int128>(x)
+                              : static_cast<unsigned __int128>(x);
+    if (m == 0) return 0.0;
+    int L = 127;
+    while (!((m >> L) & 1)) --L;  // leading bit
+    double r;
+    if (L <= 52) {
+        r = static_cast<double>(static_cast<uint64_t>(m));
+    } else {
+        const int sh = L - 52;
+        uint64_t mant = static_cast<uint64_t>(m >> sh);
+        const unsigned __int128 rem = m & ((static_cast<unsigned __int128>(1) << sh) - 1);
+        const unsigned __int128 half = static_cast<unsigned __int128>(1) << (sh - 1);
+        if (rem > half || (rem == half && (mant & 1))) ++mant;
+        r = std::ldexp(static_cast<double>(mant), sh);  // mant <= 2^53: exact
+    }
+    return neg ? -r : r;
+}
+
+template <class T> struct ExactSplitMerge {
+    const T* v;
+    int64_t n;
+    int32_t m;
+    static constexpr double MODEL_BYTES = 80.0;
+
+    static __int128 wide(T x) {
+        return std::is_signed<T>::value ? static_cast<__int128>(static_cast<int64_t>(x))
+                                        : static_cast<__int128>(static_cast<uint64_t>(x));
+    }
+    // bits and cost of a linear fit over [s, e)
+    void fit(int64_t s, int64_t e, int32_t& bits, double& cost) const {
+        const int64_t len = e - s;
+        __int128 Sy = 0, Sxy = 0;
+        for (int64_t i = 0; i < len; ++i) {
+            const __int128 y = wide(v[s + i]);
+            Sy += y;
+            Sxy += y * i;
+        }
+        const __int128 N = len;
+        const __int128 Sxx = (N - 1) * N * (2 * N - 1) / 6;
+        const int32_t nn1 = static_cast<int32_t>(static_cast<uint32_t>(len) * static_cast<uint32_t>(len - 1));
+        const double sx = static_cast<double>(nn1) / 2.0;
+        const double sy = i128_to_double(Sy), sxy = i128_to_double(Sxy), sxx = i128_to_double(Sxx);
+        const double dn = static_cast<double>(len);
+        const double det = dn * sxx - sx * sx;
+        double t0, t1;
+        if (std::fabs(det) > 1e-10) {
+            t1 = (dn * sxy - sx * sy) / det;
+            t0 = (sy - t1 * sx) / dn;
+        } else {
+            t1 = 0.0;
+            t0 = sy / dn;
+        }
+        int64_t mx = 0;
+        for (int64_t i = 0; i < len; ++i) {
+            const double pr = t0 + t1 * static_cast<double>(i);
+            const int64_t d = calc_delta<T>(v[s + i], pred_to_T<T>(pr));
+            const int64_t a = d < 0 ? static_cast<int64_t>(0ULL - static_cast<uint64_t>(d)) : d;
+            mx = std::max(mx, a);
+        }
+        bits = bits_for_max_error(mx);
+        cost = MODEL_BYTES + static_cast<double>(len) * bits / 8.0;
+    }
+    double cost(int64_t s, int64_t e) const {
+        int32_t b;
+        double c;
+        fit(s, e, b, c);
+        return c;
+    }
+    // candidate sizes in units of m, CpuPartitioner.cuh:150-194
+    static int candidates(int32_t* k) {
+        int c = 0;
+        int32_t size = 1, step = 1;
+        while (size <= 32) {
+            k[c++] = size;
+            if (size > 4) step = 2;
+            if (size > 16) step = 4;
+            size += step;
+        }
+        return c;
+    }
+    int64_t split_end(int64_t cur, double& best_cost) const {
+        int32_t k[32];
+        const int K = candidates(k);
+        int64_t best_end = std::min<int64_t>(cur + m, n);
+        double best = std::numeric_limits<double>::max();
+        const int64_t max_size = std::min<int64_t>(n - cur, static_cast<int64_t>(m) * 32);
+        for (int j = 0; j < K; ++j) {
+            const int64_t size = static_cast<int64_t>(k[j]) * m;
+            if (size > max_size) break;
+            const double cpe = cost(cur, cur + size) / static_cast<double>(size);
+            if (cpe < best) {
+                best = cpe;
+                best_end = cur + size;
+            }
+        }
+        best_cost = cost(cur, best_end);
+        return best_end;
+    }
+    void run(std::vector<int32_t>& st, std::vector<int32_t>& en) {
+        st.clear();
+        en.clear();
+        if (n <= 0) return;
+        struct Cand {
+            int64_t s, e;
+            double cost;
+        };
+        std::vector<Cand> c;
+        for (int64_t cur = 0; cur < n;) {
+            double bc;
+            const int64_t e = split_end(cur, bc);
+            c.push_back({cur, e, bc});
+            cur = e;
+        }
+        bool changed = true;
+        for (int iter = 0; changed && iter < 10; ++iter) {
+            changed = false;
+            std::vector<Cand> out;
+            Cand g = c[0];
+            for (size_t i = 1; i < c.size(); ++i) {
+                const double merged = cost(g.s, c[i].e);
+                if (merged < g.cost + c[i].cost) {
+                    g.e = c[i].e;
+                    g.cost = merged;
+                    changed = true;
+                } else {
+                    out.push_back(g);
+                    g = c[i];
+                }
+            }
+            out.push_back(g);
+            c.swap(out);
+        }
+        for (auto& x : c) {
+            st.push_back(static_cast<int32_t>(x.s));
+            en.push_back(static_cast<int32_t>(x.e));
+        }
+    }
+};
+
 // ---------------------------------------------------------------------------
 // compress() driver: api/G-LeCo.cuh:349-614
 // ---------------------------------------------------------------------------
@@ -664,7 +824,10 @@ template <class T> Col* compress_t(const T* v, int64_t n, int mode, int32_t size
     c->n = n;
     if (mode == 0) fixed_partitions(n, size_arg, c->start, c->end);
     else if (mode == 1) v6_partitions<T>(v, n, base, c->start, c->end, c->v6_var, c->v6_thr);
-    else {
+    else if (mode == 3) {
+        ExactSplitMerge<T> sm{v, n, size_arg};
+        sm.run(c->start, c->end);
+    } else {
         CpuPartitioner<T> cp{v, n, size_arg, {}};
         cp.run(c->start, c->end);
     }
@@ -891,6 +1054,24 @@ uint64_t orc_decode_pred(int dtype, int32_t type, const double* prm, int64_t i) 
     });
 }
 int32_t orc_bits_for_max_error(int64_t m) { return bits_for_max_error(m); }
+// i128_to_double of hi*2^64 + lo (two's complement)
+double orc_i128_to_double(int64_t hi, uint64_t lo) {
+    const __int128 x = static_cast<__int128>((static_cast<unsigned __int128>(static_cast<uint64_t>(hi)) << 64) | lo);
+    return i128_to_double(x);
+}
+// cost of [s, e) under the split/merge cost model: mode 2 = CpuPartitioner
+// (double prefix sums), mode 3 = exact sums.
+double orc_sm_cost(int dtype, const void* v, int64_t n, int mode, int64_t s, int64_t e) {
+    return dispatch(dtype, [&](auto t) -> double {
+        using T = decltype(t);
+        const T* x = static_cast<const T*>(v);
+        if (mode == 3) return ExactSplitMerge<T>{x, n, 128}.cost(s, e);
+        CpuPartitioner<T> cp{x, n, 128, {}};
+        cp.py.assign(static_cast<size_t>(n) + 1, 0.0);
+        for (int64_t i = 0; i < n; ++i) cp.py[i + 1] = cp.py[i] + static_cast<double>(x[i]);
+        return cp.cost(static_cast<int32_t>(s), static_cast<int32_t>(e));
+    });
+}
 uint64_t orc_gather_bits(const uint32_t* w, int64_t nw, int64_t pos, int bits) { return gather_bits(w, nw, pos, bits); }
 void orc_scatter_bits(uint32_t* w, int64_t pos, int bits, uint64_t v) { scatter_bits(w, pos, bits, v); }
 int32_t orc_fit_block_size(int32_t len0) { return fit_block_size(len0); }

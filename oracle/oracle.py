@@ -17,7 +17,7 @@ DT_I32, DT_U32, DT_I64, DT_U64 = 0, 1, 2, 3
 NP_OF = {DT_I32: np.int32, DT_U32: np.uint32, DT_I64: np.int64, DT_U64: np.uint64}
 DT_OF_NP = {np.dtype(v): k for k, v in NP_OF.items()}
 MODEL_CONSTANT, MODEL_LINEAR, MODEL_POLY2, MODEL_POLY3, MODEL_DIRECT_COPY = 0, 1, 2, 3, 4
-PART_FIXED, PART_V6, PART_CPU = 0, 1, 2
+PART_FIXED, PART_V6, PART_CPU, PART_SPLITMERGE_EXACT = 0, 1, 2, 3
 
 _lib = None
 
@@ -66,6 +66,10 @@ def lib() -> C.CDLL:
         L.orc_scatter_bits.argtypes = [vp, i64, C.c_int, C.c_uint64]
         L.orc_fit_block_size.restype = i32
         L.orc_fit_block_size.argtypes = [i32]
+        L.orc_i128_to_double.restype = C.c_double
+        L.orc_i128_to_double.argtypes = [i64, C.c_uint64]
+        L.orc_sm_cost.restype = C.c_double
+        L.orc_sm_cost.argtypes = [C.c_int, vp, i64, C.c_int, i64, i64]
         L.orc_v6_block_stat.restype = C.c_float
         L.orc_v6_block_stat.argtypes = [C.c_int, vp, i64]
         L.orc_max_threads.restype = C.c_int
@@ -258,3 +262,18 @@ def ref_checksum(b: bytes, sse42: bool = False):
     L.ref_calculate_checksum.argtypes = [C.c_void_p, C.c_size_t]
     a = np.frombuffer(b, np.uint8)
     return L.ref_calculate_checksum(_p(a), a.size)
+
+
+def i128_to_double(x: int) -> float:
+    """The oracle's round-to-nearest-even conversion of a signed 128-bit integer."""
+    u = x & ((1 << 128) - 1)
+    hi = u >> 64
+    if hi >= 1 << 63:
+        hi -= 1 << 64
+    return lib().orc_i128_to_double(hi, u & ((1 << 64) - 1))
+
+
+def sm_cost(values: np.ndarray, s: int, e: int, exact: bool = True) -> float:
+    """Split/merge cost of [s, e): exact sums (True) or CpuPartitioner's double prefix sums."""
+    values = np.ascontiguousarray(values)
+    return lib().orc_sm_cost(dtype_code(values), _p(values), values.size, 3 if exact else 2, s, e)

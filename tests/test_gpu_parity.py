@@ -18,7 +18,7 @@ pytestmark = pytest.mark.gpu
 DT_OF = {np.dtype(np.int32): glc.I32, np.dtype(np.uint32): glc.U32, np.dtype(np.int64): glc.I64,
          np.dtype(np.uint64): glc.U64}
 PART = {"fixed": (glc.PART_FIXED, O.PART_FIXED), "v6": (glc.PART_V6, O.PART_V6),
-        "cpu": (glc.PART_CPU_SPLITMERGE, O.PART_CPU)}
+        "cpu": (glc.PART_CPU_SPLITMERGE, O.PART_CPU), "gsm": (glc.PART_GPU_SPLITMERGE, O.PART_SPLITMERGE_EXACT)}
 
 
 def to_dev(v: np.ndarray):
@@ -107,11 +107,33 @@ GENS = {
 
 
 @pytest.mark.parametrize("gen", sorted(GENS))
-@pytest.mark.parametrize("part,size", [("fixed", 4096), ("fixed", 1024), ("fixed", 1000), ("v6", 0), ("cpu", 128)])
+@pytest.mark.parametrize("part,size", [("fixed", 4096), ("fixed", 1024), ("fixed", 1000), ("v6", 0), ("cpu", 128),
+                                       ("gsm", 128)])
 def test_parity_small(gpu, gen, part, size):
     n = 50_000
     v = GENS[gen](n)
     full_case(v, part, size or 4096, fixed_size=(size if part == "fixed" else None))
+
+
+def test_gpu_splitmerge_boundaries(gpu):
+    # exact regime (column prefix sums below 2^53): GPU split/merge == the
+    # reference's CPU partitioner restated; otherwise == the exact-sum oracle
+    cases = [(W.c1_int32(1 << 18), 128, O.PART_CPU),
+             (np.where(np.arange(1 << 16) < 30000, np.arange(1 << 16) * 5, 100000 - np.arange(1 << 16)).astype(np.int32),
+              128, O.PART_CPU),
+             (W.c1_int32(70000), 100, O.PART_CPU),
+             (W.c2_uint64(1 << 17, seed=4), 128, O.PART_SPLITMERGE_EXACT),
+             (W.c3_int64(1 << 17, seed=5), 256, O.PART_SPLITMERGE_EXACT),
+             (np.arange(300, dtype=np.uint32) * 7, 128, O.PART_CPU),
+             (np.array([5], dtype=np.int64), 128, O.PART_CPU)]
+    for v, m, mode in cases:
+        print("split/merge case", v.dtype, v.size, m, flush=True)
+        st, en = glc.partition(to_dev(v), glc.PART_GPU_SPLITMERGE, m)
+        ref = O.compress(v, mode, m)
+        assert np.array_equal(st, ref.start) and np.array_equal(en, ref.end), (v.dtype, v.size, m)
+        if mode == O.PART_CPU:
+            ex = O.compress(v, O.PART_SPLITMERGE_EXACT, m)
+            assert np.array_equal(ex.start, ref.start)
 
 
 def test_parity_c1_config(gpu):
