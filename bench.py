@@ -148,6 +148,82 @@ def bench_c4(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_c3(args):
+    """Sorted int64 timestamp-like column, N = 2^29, fixed 4096-value
+    partitions: packed against pre-unpacked full decompress and random access
+    (2^24 uniform lookups; generic and fixed-partition lookup)."""
+    import torch
+
+    import glc
+    from glc import workloads as W
+
+    n = 1 << args.log2n
+    q = 1 << args.lookups_log2
+    dev = torch.device("cuda:0")
+    log(f"generating c3 int64 column, N=2^{args.log2n}")
+    x = W.c3_int64_torch(n, device=dev)
+    col = glc.compress(x, glc.PART_FIXED, 4096)
+    torch.cuda.synchronize()
+    inf = col.info()
+    pos = torch.from_numpy(W.positions_uniform(q, n)).to(dev)
+    out = torch.empty_like(x)
+    look = torch.empty(q, dtype=torch.int64, device=dev)
+
+    def timed(fn, steps):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+        for a, c in ev:
+            a.record()
+            fn()
+            c.record()
+        torch.cuda.synchronize()
+        return sum(a.elapsed_time(c) for a, c in ev) / steps
+
+    res = {}
+    res["packed_full_ms"] = timed(lambda: col.decompress_full(glc.FULL_BITPACKED, out=out), args.steps)
+    assert torch.equal(out, x)
+    res["packed_fix_full_ms"] = timed(lambda: col.decompress_full_fixed(4096, out=out), args.steps)
+    res["packed_ra_ms"] = timed(lambda: col.random_access(pos, out=look), args.steps)
+    assert torch.equal(look, x[pos.long()])
+    res["packed_ra_fixed_ms"] = timed(lambda: col.random_access_fixed(pos, 4096, out=look), args.steps)
+    assert torch.equal(look, x[pos.long()])
+    col.preunpack(free_packed=False)
+    torch.cuda.synchronize()
+    res["plain_full_ms"] = timed(lambda: col.decompress_full(glc.FULL_PREUNPACKED, out=out), args.steps)
+    assert torch.equal(out, x)
+    res["plain_ra_ms"] = timed(lambda: col.random_access(pos, glc.RA_PREUNPACKED, out=look), args.steps)
+    res["plain_ra_fixed_ms"] = timed(lambda: col.random_access_fixed(pos, 4096, preunpacked=True, out=look),
+                                     args.steps)
+    assert torch.equal(look, x[pos.long()])
+    P = inf.num_partitions
+    alg_packed = (inf.total_bits + 7) // 8 + META_BYTES_PER_PARTITION * P + 8 * n
+    alg_plain = 8 * n + META_BYTES_PER_PARTITION * P + 8 * n
+    line = {
+        "metric": "c3 packed vs pre-unpacked: full-decompress GB/s and random-access Mlookups/s",
+        "value": round(8 * n / (res["packed_full_ms"] * 1e-3) / 1e9, 2), "unit": "GB/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(res["packed_full_ms"], 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int64",
+        "data": "synthetic (seeded, generated on the GPU)",
+        "config": {"workload": f"c3_int64_sorted_timestamps_N2^{args.log2n}_fixed4096", "n": n, "partitions": P,
+                   "compressed_bytes": inf.compressed_size, "lookups": q, "lookup_mix": "uniform"},
+        "ms": {k: round(v, 4) for k, v in res.items()},
+        "full_GBps": {"packed": round(8 * n / res["packed_full_ms"] / 1e6, 1),
+                      "packed_fix": round(8 * n / res["packed_fix_full_ms"] / 1e6, 1),
+                      "preunpacked": round(8 * n / res["plain_full_ms"] / 1e6, 1)},
+        "lookup_Mlookups_per_s": {k: round(q / (res[k] * 1e-3) / 1e6, 1)
+                                  for k in ("packed_ra_ms", "packed_ra_fixed_ms", "plain_ra_ms", "plain_ra_fixed_ms")},
+        "roofline": {"bound": "hbm", "achieved": round(alg_packed / (res["packed_full_ms"] * 1e-3) / 1e9, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(alg_packed / (res["packed_full_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "traffic": None, "kernel": "k_decode_wave (int64, bit-packed)", "algorithmic_bytes": alg_packed,
+                     "preunpacked_frac": round(alg_plain / (res["plain_full_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -158,7 +234,9 @@ def main():
     ap.add_argument("--mode", default="worksteal", choices=["worksteal", "bitpacked", "v2", "worksteal_adv"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-log2", type=int, default=24)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c4"])
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4"])
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run a plain device copy of the input (known bytes) for counter calibration")
     ap.add_argument("--c4-columns", type=int, default=10_000)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -170,6 +248,10 @@ def main():
         if args.log2n == 28:
             args.log2n = 30
         return bench_c4(args)
+    if args.workload == "c3":
+        if args.log2n == 28:
+            args.log2n = 29
+        return bench_c3(args)
 
     import numpy as np
     import torch
@@ -228,6 +310,15 @@ def main():
     p = lambda f: look_sorted[min(len(look_sorted) - 1, int(round(f * (len(look_sorted) - 1))))]
     look_avg = sum(look_ms) / len(look_ms)
 
+    if args.copy_probe:
+        # a plain copy of the 2 GiB input: reads and writes n*8 bytes, the
+        # yardstick for FETCH_SIZE / WRITE_SIZE of the decode kernel
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        del y
+
     traffic = None
     tf = os.path.join(ROOT, "profiles", "decode_traffic.json")
     if os.path.exists(tf):
@@ -266,7 +357,7 @@ def main():
         "compress_ms": round(compress_ms, 2),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": f"k_decode ({args.mode}, uint64, bit-packed)", "algorithmic_bytes": alg_bytes},
+                     "kernel": "k_decode_wave<uint64, packed>", "algorithmic_bytes": alg_bytes},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -90,8 +90,8 @@ template <typename T> void run(const char* tname, size_t n, size_t nq) {
         report("decompressFullFile_WorkStealing", same(out, data));
         leco.decompressFullFile_WorkStealingAdvanced(cd, out);
         report("decompressFullFile_WorkStealingAdvanced", same(out, data));
-        leco.decompressFullFileCooperative(cd, out);
-        report("decompressFullFileCooperative (FP64)", same(out, data));
+        leco.decompressFullFileCooperative(cd, out);  // needs pre-unpacked deltas (G:284-290)
+        report("decompressFullFileCooperative refuses packed-only data", out.empty());
 
         SerializedData* s1 = leco.serialize(cd);
         SerializedData* s2 = leco.serializeGPU(cd);
@@ -111,6 +111,10 @@ template <typename T> void run(const char* tname, size_t n, size_t nq) {
                 report("decompress on pre-unpacked data", same(out, want));
                 leco.decompressFullFile_PreUnpacked(d2, out);
                 report("decompressFullFile_PreUnpacked", same(out, data));
+                leco.decompressFullFileCooperative(d2, out);
+                report("decompressFullFileCooperative (FP64)", same(out, data));
+                leco.decompressFullFile(d2, out);
+                report("decompressFullFile on pre-unpacked data", same(out, data));
                 if (!c.var) {
                     leco.randomAccessFixedPreUnpacked(d2, pos, out, TILE_SIZE);
                     report("randomAccessFixedPreUnpacked", same(out, want));

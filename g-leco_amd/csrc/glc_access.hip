@@ -182,8 +182,10 @@ int launch_serve_lookup(const ServeReq* d_reqs, int32_t R, int64_t Q, hipStream_
 int launch_random_access(const glc_column* c, const void* d_pos, int pos64, int64_t q, void* d_out, int mode,
                          int32_t fixed_size, hipStream_t s) {
     if (q == 0) return GLC_OK;
-    const bool plain = (mode == GLC_RA_PREUNPACKED);
     if (mode != GLC_RA_GENERIC && mode != GLC_RA_PREUNPACKED) return fail(GLC_ERR_INVALID_ARG, "bad random-access mode");
+    // the generic lookup reads pre-unpacked deltas when the column has them
+    // (RandomAccessKernels.cuh:116-150 checks d_plain_deltas first)
+    const bool plain = (mode == GLC_RA_PREUNPACKED) || (fixed_size == 0 && c->d_plain != nullptr);
     if (plain && !c->d_plain)
         return fail(GLC_ERR_UNSUPPORTED, "no pre-unpacked deltas (call glc_preunpack or deserialize with preunpack)");
     if (!plain && !c->d_words && c->total_bits > 0)

@@ -31,7 +31,8 @@ constexpr int kDecodeThreads = 256;             // 4 wave64 per workgroup
 constexpr int kDecodeTile = 2048;               // elements per workgroup tile (8 per thread)
 constexpr int kMaxTileParts = 64;               // partitions of one tile staged in LDS (slow path)
 constexpr int kModeLdsTiles = 100;              // extra full-decode schedules (measurement): LDS workgroup tiles
-constexpr int kModeLdsSteal = 101;              //   ... and their persistent work-stealing form
+constexpr int kModeLdsSteal = 101;
+constexpr int kModeWaveSteal = 102;  // wave tiles claimed from per-XCD atomic queues (measurement)              //   ... and their persistent work-stealing form
 
 enum { MODEL_CONSTANT = 0, MODEL_LINEAR = 1, MODEL_POLY2 = 2, MODEL_POLY3 = 3, MODEL_DIRECT_COPY = 4 };
 

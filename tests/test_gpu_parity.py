@@ -47,7 +47,10 @@ def assert_same_column(col, ref, check_fit_block=True):
 
 def check_decoders(col, v, ref, fixed_size=None, q=4000, seed=0):
     n = v.size
-    for mode in (glc.FULL_AUTO, glc.FULL_BITPACKED, glc.FULL_V2, glc.FULL_WORKSTEAL, glc.FULL_WORKSTEAL_ADV):
+    # 100, 101, 102: the LDS-tile, LDS work-stealing and atomic-claim wave
+    # schedules kept for measurement
+    for mode in (glc.FULL_AUTO, glc.FULL_BITPACKED, glc.FULL_V2, glc.FULL_WORKSTEAL, glc.FULL_WORKSTEAL_ADV,
+                 100, 101, 102):
         out = host(col.decompress_full(mode))
         assert np.array_equal(out, v), f"full mode {mode}"
     rng = np.random.default_rng(seed)
