@@ -167,30 +167,76 @@ struct LenBits {
     }
 };
 
-template <class F>
-__global__ __launch_bounds__(kScanThreads) void k_scan(F f, int64_t P, int64_t* __restrict__ out,
-                                                       int64_t* __restrict__ total) {
-    __shared__ int64_t s[kScanThreads];
-    const int tid = threadIdx.x;
-    const int64_t chunk = (P + kScanThreads - 1) / kScanThreads;
-    const int64_t b0 = tid * chunk;
-    const int64_t b1 = (b0 + chunk < P) ? b0 + chunk : P;
-    int64_t local = 0;
-    for (int64_t p = b0; p < b1; ++p) local += f(p);
-    s[tid] = local;
+// Exclusive scan of f(0..P-1) into out[] and the sum into *total, in three
+// launches of 1024-value tiles (all loads coalesced).  out[] doubles as the
+// scratch for the tile sums: tile b's sum is parked in out[b * 1024] until
+// the down-sweep overwrites it with the final value.
+__device__ __forceinline__ int64_t block_excl_scan(int64_t x, int64_t* s_w, int64_t* total) {
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    int64_t v = x;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int64_t y = __shfl_up(v, o, 64);
+        if (lane >= o) v += y;
+    }
+    if (lane == 63) s_w[w] = v;
     __syncthreads();
-    for (int o = 1; o < kScanThreads; o <<= 1) {
-        int64_t y = (tid >= o) ? s[tid - o] : 0;
-        __syncthreads();
-        s[tid] += y;
-        __syncthreads();
+    if (tid == 0) {
+        int64_t acc = 0;
+        for (int k = 0; k < kScanThreads / 64; ++k) {
+            const int64_t t = s_w[k];
+            s_w[k] = acc;
+            acc += t;
+        }
+        s_w[kScanThreads / 64] = acc;
     }
-    int64_t acc = s[tid] - local;  // exclusive prefix of this chunk
-    for (int64_t p = b0; p < b1; ++p) {
-        out[p] = acc;
-        acc += f(p);
+    __syncthreads();
+    const int64_t r = s_w[w] + v - x;
+    *total = s_w[kScanThreads / 64];
+    __syncthreads();
+    return r;
+}
+
+template <class F> __global__ __launch_bounds__(kScanThreads) void k_scan_up(F f, int64_t P, int64_t* __restrict__ out) {
+    __shared__ int64_t s_w[kScanThreads / 64 + 1];
+    const int64_t p = static_cast<int64_t>(blockIdx.x) * kScanThreads + threadIdx.x;
+    int64_t tot;
+    (void)block_excl_scan(p < P ? f(p) : 0, s_w, &tot);
+    if (threadIdx.x == 0) out[static_cast<int64_t>(blockIdx.x) * kScanThreads] = tot;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_mid(int64_t* __restrict__ out, int64_t nb,
+                                                           int64_t* __restrict__ total) {
+    __shared__ int64_t s_w[kScanThreads / 64 + 1];
+    int64_t carry = 0;
+    for (int64_t b0 = 0; b0 < nb; b0 += kScanThreads) {
+        const int64_t b = b0 + threadIdx.x;
+        const int64_t x = b < nb ? out[b * kScanThreads] : 0;
+        int64_t tot;
+        const int64_t e = block_excl_scan(x, s_w, &tot);
+        if (b < nb) out[b * kScanThreads] = carry + e;
+        carry += tot;
     }
-    if (tid == kScanThreads - 1) *total = s[tid];
+    if (threadIdx.x == 0) *total = carry;
+}
+
+template <class F> __global__ __launch_bounds__(kScanThreads) void k_scan_down(F f, int64_t P, int64_t* __restrict__ out) {
+    __shared__ int64_t s_w[kScanThreads / 64 + 1];
+    const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanThreads;
+    const int64_t p = base + threadIdx.x;
+    const int64_t carry = out[base];
+    int64_t tot;
+    const int64_t e = block_excl_scan(p < P ? f(p) : 0, s_w, &tot);  // all threads have read carry
+    if (p < P) out[p] = carry + e;
+}
+
+template <class F> int launch_scan(F f, int64_t P, int64_t* out, int64_t* total, hipStream_t s) {
+    if (P <= 0) return fail(GLC_ERR_STATE, "empty scan");
+    const int64_t nb = (P + kScanThreads - 1) / kScanThreads;
+    hipLaunchKernelGGL(k_scan_up<F>, dim3(static_cast<unsigned>(nb)), dim3(kScanThreads), 0, s, f, P, out);
+    hipLaunchKernelGGL(k_scan_mid, dim3(1), dim3(kScanThreads), 0, s, out, nb, total);
+    hipLaunchKernelGGL(k_scan_down<F>, dim3(static_cast<unsigned>(nb)), dim3(kScanThreads), 0, s, f, P, out);
+    GLC_HIP(hipGetLastError());
+    return GLC_OK;
 }
 
 // Batched offsets: one workgroup per column scans that column's partitions
@@ -603,7 +649,8 @@ int launch_fit(int dtype, const void* d_values, glc_column* c, int32_t bs, hipSt
 
 int launch_scan_offsets(glc_column* c, int64_t* d_total, hipStream_t s) {
     LenBits f{c->d_start, c->d_end, c->d_bits};
-    hipLaunchKernelGGL(k_scan<LenBits>, dim3(1), dim3(kScanThreads), 0, s, f, c->P, c->d_off, d_total);
+    int rc = launch_scan(f, c->P, c->d_off, d_total, s);
+    if (rc) return rc;
     GLC_HIP(hipGetLastError());
     return GLC_OK;
 }
@@ -665,7 +712,8 @@ int launch_v6(int dtype, const void* d_values, int64_t n, int32_t base, int32_t*
     V6Count f{d_var, d_thr, vbs, n, base};
     // per-block partition base index (exclusive scan) stored after the counts slot
     int64_t* d_base = d_count + 1;
-    hipLaunchKernelGGL(k_scan<V6Count>, dim3(1), dim3(kScanThreads), 0, s, f, nvb, d_base, d_count);
+    rc = launch_scan(f, nvb, d_base, d_count, s);
+    if (rc) return rc;
     GLC_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_v6_emit, dim3(grid_for(nvb, 256)), dim3(256), 0, s, f, nvb, d_base, d_start, d_end);
     GLC_HIP(hipGetLastError());

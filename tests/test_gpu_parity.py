@@ -251,7 +251,9 @@ def test_empty_column_and_errors(gpu):
     with pytest.raises(glc.GlcError):
         col.random_access_fixed(to_dev(np.zeros(4, np.int32)), 512)
     with pytest.raises(glc.GlcError):
-        glc.compress(to_dev(v), glc.PART_GPU_SPLITMERGE)
+        glc.compress(to_dev(v), 99)  # unknown partitioner
+    with pytest.raises(glc.GlcError):
+        glc.compress(to_dev(v), glc.PART_FIXED, -5)
     col.preunpack(free_packed=True)
     assert np.array_equal(host(col.decompress_full()), v)
     with pytest.raises(glc.GlcError):
