@@ -18,6 +18,7 @@ decoding a bounded prefix sample of the same column on the host's cores.
 """
 import argparse
 import json
+import math
 import os
 import statistics
 import sys
@@ -84,12 +85,12 @@ def bench_c4(args):
     raw_bytes = sum(t.numel() * t.element_size() for t in dev)
     torch.cuda.synchronize()
     b = glc.Batch(dev, 1024)
-    outs = [torch.empty_like(t) for t in dev]
+    outs = b.set_outputs([torch.empty_like(t) for t in dev])  # pointer table built once
     b.compress()
-    (w, t), tune_ms = b.autotune(outs, reps=3)
+    (w, t), tune_ms = b.autotune(reps=3)
     log(f"autotuned schedule {w}x{t} ({tune_ms:.3f} ms)")
     for _ in range(args.warmup):
-        b.run_graph(glc.BATCH_COMPRESS | glc.BATCH_DECOMPRESS, outs)
+        b.run_graph(glc.BATCH_COMPRESS | glc.BATCH_DECOMPRESS)
     torch.cuda.synchronize()
     for x, o in zip(dev, outs):
         assert torch.equal(x, o), "batch round trip mismatch"
@@ -105,8 +106,8 @@ def bench_c4(args):
         torch.cuda.synchronize()
         return sorted(a.elapsed_time(c) for a, c in ev)
 
-    both = timed(lambda: b.run_graph(glc.BATCH_COMPRESS | glc.BATCH_DECOMPRESS, outs), args.steps)
-    dec = timed(lambda: b.run_graph(glc.BATCH_DECOMPRESS, outs), args.steps)
+    both = timed(lambda: b.run_graph(glc.BATCH_COMPRESS | glc.BATCH_DECOMPRESS), args.steps)
+    dec = timed(lambda: b.run_graph(glc.BATCH_DECOMPRESS), args.steps)
     # per-column loop of the plain API
     log("per-column loop baseline")
     t0 = time.perf_counter()
@@ -119,8 +120,18 @@ def bench_c4(args):
     torch.cuda.synchronize()
     for x, o in zip(dev, outs):
         assert torch.equal(x, o), "per-column round trip mismatch"
-    loop_dec = timed(lambda: [c.decompress_full(glc.FULL_BITPACKED, out=o) for c, o in zip(cols, outs)],
-                     max(3, args.steps // 10))
+    # the loop calls the C-ABI directly with prebuilt arguments (no per-call
+    # Python object work), so it measures launch + kernel per column
+    import ctypes as C
+    fn = glc.lib().glc_decompress_full
+    sh = glc._stream(None)
+    calls = [(c.handle, C.c_void_p(o.data_ptr())) for c, o in zip(cols, outs)]
+
+    def loop():
+        for h, p in calls:
+            fn(h, p, glc.FULL_BITPACKED, sh)
+
+    loop_dec = timed(loop, max(3, args.steps // 2))
     med = lambda a: a[len(a) // 2]
     p99 = lambda a: a[min(len(a) - 1, int(round(0.99 * (len(a) - 1))))]
     dec_ms = sum(dec) / len(dec)
@@ -224,17 +235,177 @@ def bench_c3(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_c5(args):
+    """Serving form: resident compressed columns (half from the uint64
+    piecewise-linear generator with V6, half from the int64 timestamp
+    generator with fixed 4096-value partitions; lengths log-uniform in
+    [2^20, 2^26]).  A Poisson stream of requests -- 95% lookup batches of
+    log-uniform size in [1, 2^20] (Zipf(1.1) positions, Zipf(1.1) over
+    columns), 5% whole-column decompressions -- is served (a) by the plain
+    per-request path (one request at a time: positions to the device, one
+    lookup or decode launch, wait) and (b) by the serving loop (HBM-budget
+    admission, continuous batching).  Latency = arrival -> results in HBM."""
+    import threading
+
+    import numpy as np
+    import torch
+
+    import glc
+    from glc import workloads as W
+
+    rng = np.random.default_rng(1234)
+    C = args.c5_columns
+    lo, hi = args.c5_min_log2, args.c5_max_log2
+    lens = np.exp(rng.uniform(math.log(2 ** lo), math.log(2 ** hi), C)).astype(np.int64)
+    log(f"building {C} resident columns ({int(lens.sum())} values)")
+    cols = []
+    comp = 0
+    for j in range(C):
+        n = int(lens[j])
+        if j % 2 == 0:
+            x = W.c2_uint64_torch(n, seed=1000 + j)
+            c = glc.compress(x, glc.PART_V6, 0, 2048)
+        else:
+            x = W.c3_int64_torch(n, seed=1000 + j)
+            c = glc.compress(x, glc.PART_FIXED, 4096)
+        comp += c.info().compressed_size
+        cols.append(c)
+        del x
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    log(f"resident: {comp / 2**30:.2f} GiB compressed")
+
+    # request stream (positions precomputed so the clock measures serving only)
+    zr = W.zipf_ranks(1 << 20, 1 << 20, seed=77)
+    col_rank = W.zipf_ranks(args.c5_requests, C, seed=78)
+    col_of = W.scramble(col_rank, C)
+    reqs = []
+    for k in range(args.c5_requests):
+        c = int(col_of[k])
+        if rng.random() < 0.05:
+            reqs.append(("dec", c, None))
+            continue
+        q = int(np.exp(rng.uniform(0.0, math.log(2 ** 20))))
+        off = int(rng.integers(0, (1 << 20) - q + 1))
+        r = zr[off:off + q].astype(np.uint64)
+        n = int(lens[c])
+        pos = ((r * np.uint64(2654435761)) % np.uint64(n)).astype(np.int32)
+        reqs.append(("look", c, pos))
+    nlook = sum(r[2].size for r in reqs if r[0] == "look")
+    log(f"{len(reqs)} requests: {sum(r[0] == 'dec' for r in reqs)} decompress, {nlook} lookups")
+
+    def plain(lam, reqs):
+        arr = np.cumsum(rng.exponential(1.0 / lam, len(reqs))) if lam else np.zeros(len(reqs))
+        lat = []
+        t0 = time.perf_counter()
+        for k, (kind, c, pos) in enumerate(reqs):
+            ta = t0 + arr[k]
+            while time.perf_counter() < ta:
+                pass
+            col = cols[c]
+            if kind == "dec":
+                col.decompress_full(glc.FULL_AUTO)
+            else:
+                col.random_access(torch.from_numpy(pos).cuda())
+            torch.cuda.synchronize()
+            lat.append(time.perf_counter() - ta)
+        wall = time.perf_counter() - t0
+        lat = sorted(lat)
+        return {"p50_ms": 1e3 * lat[len(lat) // 2], "p99_ms": 1e3 * lat[int(0.99 * (len(lat) - 1))],
+                "req_per_s": len(reqs) / wall, "offered_req_per_s": lam}
+
+    budget = int(args.c5_budget_gib * 2 ** 30)
+    srv = glc.Server(budget, max_inflight=args.c5_max_inflight)
+    ids = [srv.add_column(c) for c in cols]
+
+    def served(lam, reqs):
+        arr = np.cumsum(rng.exponential(1.0 / lam, len(reqs)))
+        rid = [None] * len(reqs)
+        srv.reset_stats()
+        t0 = time.perf_counter()
+
+        def client():
+            for k, (kind, c, pos) in enumerate(reqs):
+                ta = t0 + arr[k]
+                while time.perf_counter() < ta:
+                    time.sleep(0)
+                rid[k] = srv.submit_decompress(ids[c]) if kind == "dec" else srv.submit_lookup(ids[c], pos)
+
+        th = threading.Thread(target=client)
+        th.start()
+        for k in range(len(reqs)):
+            while rid[k] is None:
+                time.sleep(20e-6)
+            srv.wait(rid[k], 120000)
+            srv.release(rid[k])
+        th.join()
+        wall = time.perf_counter() - t0
+        st = srv.stats()
+        return {"p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "p99_lookup_ms": st["p99_lookup_ms"],
+                "p99_decompress_ms": st["p99_decompress_ms"], "req_per_s": len(reqs) / wall,
+                "offered_req_per_s": lam, "mean_requests_per_step": st["mean_requests_per_step"],
+                "inflight_peak": st["inflight_peak"], "pool_peak_gib": st["pool_peak_bytes"] / 2**30}
+
+    # equality spot check: a few results through both paths
+    for k in range(0, len(reqs), max(1, len(reqs) // 8)):
+        kind, c, pos = reqs[k]
+        a = (cols[c].decompress_full() if kind == "dec" else cols[c].random_access(torch.from_numpy(pos).cuda()))
+        r = srv.submit_decompress(ids[c]) if kind == "dec" else srv.submit_lookup(ids[c], pos)
+        srv.wait(r)
+        assert np.array_equal(srv.result(r, ids[c]), a.cpu().numpy()), "served result differs"
+        srv.release(r)
+    log("served results equal the plain path; warming up")
+    plain(0, reqs[:20])
+    served(1e5, reqs[:50])
+
+    cap = plain(0, reqs[: args.c5_probe])  # back-to-back: the plain path's capacity
+    mu = cap["req_per_s"]
+    log(f"plain capacity {mu:.1f} req/s (p50 {cap['p50_ms']:.2f} ms)")
+    rows = {"plain_closed_loop": cap}
+    for f in (0.5, 0.8):
+        rows[f"plain_{f}mu"] = plain(f * mu, reqs)
+        log(f"plain at {f}mu: {rows[f'plain_{f}mu']}")
+    for f in (0.5, 0.8, 1.5, 3.0, 6.0):
+        rows[f"served_{f}mu"] = served(f * mu, reqs)
+        log(f"served at {f}mu: {rows[f'served_{f}mu']}")
+    ref_p99 = rows["plain_0.8mu"]["p99_ms"]
+    ok = [v for k, v in rows.items() if k.startswith("served") and v["p99_ms"] <= ref_p99]
+    best = max(ok, key=lambda v: v["req_per_s"]) if ok else None
+    line = {
+        "metric": "serving: requests/s at p99 latency no worse than the plain path's at 0.8 of its capacity",
+        "value": round(best["req_per_s"], 1) if best else None, "unit": "req/s", "n_gpus": 1,
+        "steps": len(reqs), "warmup": 50, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u64/i64", "data": "synthetic (seeded, generated on the GPU)",
+        "config": {"workload": f"c5_serving_{C}_columns_2^{lo}..2^{hi}", "columns": C, "values": int(lens.sum()),
+                   "compressed_gib": round(comp / 2**30, 3), "requests": len(reqs), "lookups": int(nlook),
+                   "budget_gib": args.c5_budget_gib, "max_inflight": args.c5_max_inflight},
+        "plain_capacity_req_per_s": round(mu, 1),
+        "speedup_vs_plain_at_equal_p99": round(best["req_per_s"] / rows["plain_0.8mu"]["req_per_s"], 3) if best else None,
+        "rows": {k: {kk: (round(vv, 3) if isinstance(vv, float) else vv) for kk, vv in v.items()} for k, v in rows.items()},
+        "roofline": None, "cpu_baseline": None,
+    }
+    srv.close()
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--log2n", type=int, default=28)
     ap.add_argument("--lookups-log2", type=int, default=24)
     ap.add_argument("--mode", default="worksteal", choices=["worksteal", "bitpacked", "v2", "worksteal_adv"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-log2", type=int, default=24)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4"])
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5"])
+    ap.add_argument("--c5-columns", type=int, default=512)
+    ap.add_argument("--c5-min-log2", type=int, default=20)
+    ap.add_argument("--c5-max-log2", type=int, default=26)
+    ap.add_argument("--c5-requests", type=int, default=600)
+    ap.add_argument("--c5-probe", type=int, default=200)
+    ap.add_argument("--c5-budget-gib", type=float, default=16.0)
+    ap.add_argument("--c5-max-inflight", type=int, default=256)
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain device copy of the input (known bytes) for counter calibration")
     ap.add_argument("--c4-columns", type=int, default=10_000)
@@ -248,6 +419,8 @@ def main():
         if args.log2n == 28:
             args.log2n = 30
         return bench_c4(args)
+    if args.workload == "c5":
+        return bench_c5(args)
     if args.workload == "c3":
         if args.log2n == 28:
             args.log2n = 29

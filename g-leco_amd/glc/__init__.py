@@ -396,8 +396,15 @@ class Batch:
     def __len__(self):
         return len(self.columns)
 
+    def set_outputs(self, outs):
+        """Bind the per-column destinations once; later calls with outs=None reuse them."""
+        self._bound = self._outs(outs)
+        return self._bound[0]
+
     def _outs(self, outs):
         import torch
+        if outs is None and getattr(self, "_bound", None) is not None:
+            return self._bound
         if outs is None:
             outs = [torch.empty_like(t) for t in self.columns]
         if len(outs) != len(self.columns):
