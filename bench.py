@@ -402,7 +402,7 @@ def main():
     ap.add_argument("--c5-columns", type=int, default=512)
     ap.add_argument("--c5-min-log2", type=int, default=20)
     ap.add_argument("--c5-max-log2", type=int, default=26)
-    ap.add_argument("--c5-requests", type=int, default=600)
+    ap.add_argument("--c5-requests", type=int, default=2000)
     ap.add_argument("--c5-probe", type=int, default=200)
     ap.add_argument("--c5-budget-gib", type=float, default=16.0)
     ap.add_argument("--c5-max-inflight", type=int, default=256)
