@@ -380,11 +380,13 @@ __global__ void k_build_tinfo_batch(const BatchCol* __restrict__ cols, const int
 }
 
 // ---------------------------------------------------------------------------
-// Pack.  One workgroup per 2048-element tile.
+// Pack.  Tiles inside one partition: one wave per 1024-value tile, word-major
+// (pack_wave).  Other tiles: one workgroup assembles the tile's words in LDS
+// (pack_tile).
 // ---------------------------------------------------------------------------
 constexpr int kPackThreads = 256;
-constexpr int kPackTile = 2048;
-constexpr int kPackWords = kPackTile * 2 + 8;
+constexpr int kPackTile = 1024;                 // values per LDS-assembled tile (one index tile)
+constexpr int kPackWords = kPackTile * 2 + 8;   // 64-bit fields + the 3 words past the last one
 
 // bit position of element e's first bit, or of the next stored bit when e is
 // in a gap / past the end of the last partition with start <= e.
@@ -457,19 +459,123 @@ __device__ __forceinline__ void pack_tile(const T* __restrict__ v, const Desc* _
     __syncthreads();  // LDS reused by the next tile
 }
 
+// Wave pack of one 1024-value tile that lies inside one partition (the
+// tile's TileInfo path is not TILE_SLOW).  Phase 1: coalesced loads, one
+// residual code per value (lane l: values l, l+64, ...), parked in the
+// wave's LDS slice.  Phase 2, word-major: lane j assembles output words j,
+// j+64, ... from the (at most 32/bits + 2) codes that overlap each word, so
+// no word is shared between lanes and no atomics are needed inside the
+// tile.  The tile's first and last words may be shared with the
+// neighbouring tiles: those two are OR-ed into the zero-initialised payload
+// atomically; every other word is stored whole.
+constexpr int kPackWaves = 4;
 template <class T>
-__global__ __launch_bounds__(kPackThreads) void k_pack(const T* __restrict__ v, const Desc* __restrict__ desc,
-                                                       const int32_t* __restrict__ tile_part, int64_t n,
-                                                       uint32_t* __restrict__ words) {
-    __shared__ uint32_t s_w[kPackWords];
-    __shared__ int64_t s_ww[2];
-    const int64_t e0 = static_cast<int64_t>(blockIdx.x) * kPackTile;
-    if (e0 >= n) return;
-    const int64_t e1 = (e0 + kPackTile < n) ? e0 + kPackTile : n;
-    pack_tile<T>(v, desc, tile_part, e0, e1, words, s_w, s_ww);
+__device__ __forceinline__ void pack_wave(const T* __restrict__ v, const Desc* __restrict__ desc, const TileInfo& x,
+                                          int64_t e0, int64_t e1, uint32_t* __restrict__ words, int lane,
+                                          uint64_t* __restrict__ s_code) {
+    const int b = tile_bits(x.info);
+    if (b == 0) return;
+    const Desc d = desc[x.p];
+    const bool direct = desc_type(d.packed) == MODEL_DIRECT_COPY;
+    const int32_t lbase = static_cast<int32_t>(e0 - d.start);
+    const int32_t cnt = static_cast<int32_t>(e1 - e0);
+    const uint64_t mask = (b < 64) ? ((1ULL << b) - 1ULL) : ~0ULL;
+    const T* __restrict__ vt = v + e0;
+    T val[kIndexTile / 64];
+#pragma unroll
+    for (int r = 0; r < kIndexTile / 64; ++r) {  // all 16 loads in flight before any compute
+        const int32_t k = r * 64 + lane;
+        val[r] = vt[k < cnt ? k : cnt - 1];
+    }
+#pragma unroll
+    for (int r = 0; r < kIndexTile / 64; ++r) {
+        const int32_t k = r * 64 + lane;
+        const uint64_t code =
+            direct ? static_cast<uint64_t>(val[r])
+                   : static_cast<uint64_t>(calc_delta<T>(val[r], enc_pred<T>(d.theta0, d.theta1, lbase + k)));
+        s_code[k] = (k < cnt) ? (code & mask) : 0;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int64_t w0 = x.W >> 5;
+    const int32_t r0 = static_cast<int32_t>(x.W & 31);
+    const int32_t Wt = (r0 + cnt * b + 31) >> 5;  // <= 2049
+    for (int32_t j = lane; j < Wt; j += 64) {
+        const int32_t lo_bit = 32 * j - r0;  // first stream bit of word j (relative to element e0)
+        const int32_t k0 = lo_bit < 0 ? 0 : lo_bit / b;
+        int32_t k1 = (lo_bit + 31) / b;
+        if (k1 > cnt - 1) k1 = cnt - 1;
+        uint32_t wv = 0;
+        for (int32_t k = k0; k <= k1; ++k) {
+            const uint64_t code = s_code[k];
+            const int32_t pos = k * b - lo_bit;  // in (-b, 32)
+            wv |= (pos >= 0) ? static_cast<uint32_t>(code << pos) : static_cast<uint32_t>(code >> (-pos));
+        }
+        uint32_t* p = words + w0 + j;
+        if (j == 0 || j == Wt - 1) {
+            if (wv) atomicOr(p, wv);
+        } else {
+            *p = wv;
+        }
+    }
 }
 
-// Batched pack over 1024-element tiles of every column of type T.
+template <class T>
+__global__ __launch_bounds__(64 * kPackWaves) void k_pack_wave(const T* __restrict__ v, const Desc* __restrict__ desc,
+                                                               const TileInfo* __restrict__ tinfo, int64_t n,
+                                                               uint32_t* __restrict__ words) {
+    __shared__ uint64_t s_code[kPackWaves][kIndexTile];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * kPackWaves + wave;
+    const int64_t e0 = t * kIndexTile;
+    if (e0 >= n) return;
+    const TileInfo x = tinfo[t];
+    if (tile_path(x.info) == TILE_SLOW) return;
+    const int64_t e1 = (e0 + kIndexTile < n) ? e0 + kIndexTile : n;
+    pack_wave<T>(v, desc, x, e0, e1, words, lane, s_code[wave]);
+}
+
+// The tiles the wave pack leaves (several partitions, gaps, POLY models):
+// LDS assembly per 1024-value tile.
+template <class T>
+__global__ __launch_bounds__(kPackThreads) void k_pack_slow(const T* __restrict__ v, const Desc* __restrict__ desc,
+                                                            const int32_t* __restrict__ tile_part,
+                                                            const TileInfo* __restrict__ tinfo, int64_t n,
+                                                            int64_t ntiles, uint32_t* __restrict__ words) {
+    __shared__ uint32_t s_w[kPackWords];
+    __shared__ int64_t s_ww[2];
+    for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        if (tile_path(tinfo[t].info) != TILE_SLOW) continue;
+        const int64_t e0 = t * kIndexTile;
+        const int64_t e1 = (e0 + kIndexTile < n) ? e0 + kIndexTile : n;
+        pack_tile<T>(v, desc, tile_part, e0, e1, words, s_w, s_ww);
+    }
+}
+
+
+// Batched pack over the 1024-value tiles of every column of type T: the wave
+// pack for tiles inside one partition, LDS assembly for the others.
+template <class T>
+__global__ __launch_bounds__(64 * kPackWaves) void k_pack_wave_batch(const BatchCol* __restrict__ cols,
+                                                                     const int32_t* __restrict__ tile_col, int64_t Ttot,
+                                                                     int dtype) {
+    __shared__ uint64_t s_code[kPackWaves][kIndexTile];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t g = static_cast<int64_t>(blockIdx.x) * kPackWaves + wave;
+    if (g >= Ttot) return;
+    const BatchCol& c = cols[tile_col[g]];
+    if (c.dtype != dtype || c.payload_words > c.words_reserved) return;
+    const int64_t t = g - c.tile_base;
+    const TileInfo x = c.tinfo[t];
+    if (tile_path(x.info) == TILE_SLOW) return;
+    const int64_t e0 = t * kIndexTile;
+    const int64_t e1 = (e0 + kIndexTile < c.n) ? e0 + kIndexTile : c.n;
+    pack_wave<T>(static_cast<const T*>(c.values), c.desc, x, e0, e1, c.words, lane, s_code[wave]);
+}
+
 template <class T>
 __global__ __launch_bounds__(kPackThreads) void k_pack_batch(const BatchCol* __restrict__ cols,
                                                              const int32_t* __restrict__ tile_col, int64_t Ttot,
@@ -479,7 +585,9 @@ __global__ __launch_bounds__(kPackThreads) void k_pack_batch(const BatchCol* __r
     for (int64_t g = blockIdx.x; g < Ttot; g += gridDim.x) {
         const BatchCol& c = cols[tile_col[g]];
         if (c.dtype != dtype || c.payload_words > c.words_reserved) continue;
-        const int64_t e0 = (g - c.tile_base) * kIndexTile;
+        const int64_t t = g - c.tile_base;
+        if (tile_path(c.tinfo[t].info) != TILE_SLOW) continue;
+        const int64_t e0 = t * kIndexTile;
         const int64_t e1 = (e0 + kIndexTile < c.n) ? e0 + kIndexTile : c.n;
         pack_tile<T>(static_cast<const T*>(c.values), c.desc, c.tile_part, e0, e1, c.words, s_w, s_ww);
     }
@@ -676,8 +784,14 @@ int launch_pack(int dtype, const void* d_values, glc_column* c, hipStream_t s) {
     if (c->P == 0 || c->n == 0 || c->total_bits == 0) return GLC_OK;
     return dispatch_dtype(dtype, [&](auto t) {
         using T = decltype(t);
-        hipLaunchKernelGGL(k_pack<T>, dim3(grid_for(c->n, kPackTile)), dim3(kPackThreads), 0, s,
-                           static_cast<const T*>(d_values), c->d_desc, c->d_tile_part, c->n, c->d_words);
+        const int64_t ntiles = c->n_index - 1;
+        hipLaunchKernelGGL(k_pack_wave<T>, dim3(grid_for(ntiles, kPackWaves)), dim3(64 * kPackWaves), 0, s,
+                           static_cast<const T*>(d_values),
+                           c->d_desc, c->d_tinfo, c->n, c->d_words);
+        GLC_HIP(hipGetLastError());
+        const unsigned gs = static_cast<unsigned>(ntiles < 8192 ? ntiles : 8192);
+        hipLaunchKernelGGL(k_pack_slow<T>, dim3(gs), dim3(kPackThreads), 0, s, static_cast<const T*>(d_values),
+                           c->d_desc, c->d_tile_part, c->d_tinfo, c->n, ntiles, c->d_words);
         GLC_HIP(hipGetLastError());
         return GLC_OK;
     });
@@ -794,11 +908,13 @@ int launch_batch_compress(const BatchPlan& b, hipStream_t s) {
     GLC_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_zero_edges_batch, dim3(grid_for(b.Ttot, 256)), dim3(256), 0, s, b.cols, b.tile_col, b.Ttot);
     GLC_HIP(hipGetLastError());
-    const unsigned gp = static_cast<unsigned>(b.Ttot < (1LL << 20) ? b.Ttot : (1LL << 20));
+    const unsigned gp = static_cast<unsigned>(b.Ttot < 8192 ? b.Ttot : 8192);
     for (int dt = GLC_I32; dt <= GLC_U64; ++dt) {
         if (!b.has_dtype[dt]) continue;
         int rc = dispatch_dtype(dt, [&](auto t) {
             using T = decltype(t);
+            hipLaunchKernelGGL(k_pack_wave_batch<T>, dim3(grid_for(b.Ttot, kPackWaves)), dim3(64 * kPackWaves), 0, s,
+                               b.cols, b.tile_col, b.Ttot, dt);
             hipLaunchKernelGGL(k_pack_batch<T>, dim3(gp), dim3(kPackThreads), 0, s, b.cols, b.tile_col, b.Ttot, dt);
             GLC_HIP(hipGetLastError());
             return GLC_OK;
