@@ -26,6 +26,7 @@ constexpr int kThreads = 256;
 struct RaArgs {
     const Desc* __restrict__ desc;
     const int32_t* __restrict__ tile_part;
+    const TileInfo* __restrict__ tinfo;
     const uint32_t* __restrict__ words;
     const int64_t* __restrict__ plain;
     const double* __restrict__ params;
@@ -41,9 +42,30 @@ struct RaArgs {
 // Value at element e of a column (0 outside [0, N) and in gaps).
 template <class T, bool PLAIN>
 __device__ __forceinline__ T ra_lookup(const Desc* __restrict__ desc, const int32_t* __restrict__ tile_part,
-                                       const uint32_t* __restrict__ words, const int64_t* __restrict__ plain,
-                                       const double* __restrict__ params, int64_t n, int32_t fixed, int64_t e) {
+                                       const TileInfo* __restrict__ tinfo, const uint32_t* __restrict__ words,
+                                       const int64_t* __restrict__ plain, const double* __restrict__ params,
+                                       int64_t n, int32_t fixed, int64_t e) {
     if (e < 0 || e >= n) return T(0);
+    if (fixed <= 0) {
+        // a tile inside one partition: one 32-byte record gives model, index and bit position
+        const int64_t t = e >> kIndexShift;
+        const TileInfo x = tinfo[t];
+        if (tile_path(x.info) != TILE_SLOW) {
+            const int b = tile_bits(x.info);
+            const int32_t type = tile_type(x.info);
+            const int64_t k = e - (t << kIndexShift);
+            const int32_t i = x.lbase + static_cast<int32_t>(k);
+            if constexpr (PLAIN) {
+                const int64_t pl = plain[e];
+                if (type == MODEL_DIRECT_COPY) return static_cast<T>(pl);
+                return apply_delta<T>(dec_pred<T>(x.t0, x.t1, i), pl);
+            } else {
+                const uint64_t raw = (b > 0) ? extract_raw_g(words, x.W + k * b, b) : 0;
+                if (type == MODEL_DIRECT_COPY) return static_cast<T>(raw);
+                return apply_delta<T>(dec_pred<T>(x.t0, x.t1, i), sign_extend(raw, b));
+            }
+        }
+    }
     int64_t p;
     if (fixed > 0) {
         p = e / fixed;
@@ -83,7 +105,7 @@ __global__ __launch_bounds__(kThreads) void k_random_access(RaArgs a) {
     for (int64_t j = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x; j < a.q; j += stride) {
         const int64_t e = POS64 ? static_cast<const int64_t*>(a.pos)[j]
                                 : static_cast<int64_t>(static_cast<const int32_t*>(a.pos)[j]);
-        out[j] = ra_lookup<T, PLAIN>(a.desc, a.tile_part, a.words, a.plain, a.params, a.n, a.fixed, e);
+        out[j] = ra_lookup<T, PLAIN>(a.desc, a.tile_part, a.tinfo, a.words, a.plain, a.params, a.n, a.fixed, e);
     }
 }
 
@@ -103,19 +125,19 @@ __global__ __launch_bounds__(kThreads) void k_serve_lookup(const ServeReq* __res
         switch (r.dtype) {
             case GLC_I32:
                 static_cast<int32_t*>(r.out)[k] =
-                    ra_lookup<int32_t, false>(r.desc, r.tile_part, r.words, nullptr, r.params, r.n, r.fixed, e);
+                    ra_lookup<int32_t, false>(r.desc, r.tile_part, r.tinfo, r.words, nullptr, r.params, r.n, r.fixed, e);
                 break;
             case GLC_U32:
                 static_cast<uint32_t*>(r.out)[k] =
-                    ra_lookup<uint32_t, false>(r.desc, r.tile_part, r.words, nullptr, r.params, r.n, r.fixed, e);
+                    ra_lookup<uint32_t, false>(r.desc, r.tile_part, r.tinfo, r.words, nullptr, r.params, r.n, r.fixed, e);
                 break;
             case GLC_I64:
                 static_cast<int64_t*>(r.out)[k] =
-                    ra_lookup<int64_t, false>(r.desc, r.tile_part, r.words, nullptr, r.params, r.n, r.fixed, e);
+                    ra_lookup<int64_t, false>(r.desc, r.tile_part, r.tinfo, r.words, nullptr, r.params, r.n, r.fixed, e);
                 break;
             default:
                 static_cast<uint64_t*>(r.out)[k] =
-                    ra_lookup<uint64_t, false>(r.desc, r.tile_part, r.words, nullptr, r.params, r.n, r.fixed, e);
+                    ra_lookup<uint64_t, false>(r.desc, r.tile_part, r.tinfo, r.words, nullptr, r.params, r.n, r.fixed, e);
                 break;
         }
     }
@@ -193,6 +215,7 @@ int launch_random_access(const glc_column* c, const void* d_pos, int pos64, int6
     RaArgs a;
     a.desc = c->d_desc;
     a.tile_part = c->d_tile_part;
+    a.tinfo = c->d_tinfo;
     a.words = c->d_words;
     a.plain = c->d_plain;
     a.params = c->d_params;

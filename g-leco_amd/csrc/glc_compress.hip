@@ -319,18 +319,7 @@ __global__ void k_build_tinfo(const Desc* __restrict__ desc, const int32_t* __re
     if (t >= ntiles) return;
     const int64_t e0 = t * kIndexTile;
     const int64_t e1 = (e0 + kIndexTile < n) ? e0 + kIndexTile : n;
-    const int32_t p = tile_part[t];
-    const Desc d = desc[p];
-    const int b = desc_bits(d.packed);
-    const int type = desc_type(d.packed);
-    const bool inside = e0 >= d.start && e1 <= static_cast<int64_t>(d.start) + d.len;
-    int path = TILE_SLOW;
-    if (inside && type != MODEL_POLY2) path = (b <= 32 && type != MODEL_DIRECT_COPY) ? TILE_NARROW : TILE_WIDE;
-    TileInfo x;
-    x.W = inside ? desc_off(d.packed) + (e0 - d.start) * b : 0;
-    x.p = p;
-    x.info = b | (path << 8);
-    ti[t] = x;
+    ti[t] = make_tile_info(desc[tile_part[t]], e0, e1);
 }
 
 // Batched index: descriptors + tile_part for every partition of every column
@@ -365,18 +354,7 @@ __global__ void k_build_tinfo_batch(const BatchCol* __restrict__ cols, const int
     const int64_t t = g - c.tile_base;
     const int64_t e0 = t * kIndexTile;
     const int64_t e1 = (e0 + kIndexTile < c.n) ? e0 + kIndexTile : c.n;
-    const int32_t p = c.tile_part[t];
-    const Desc d = c.desc[p];
-    const int b = desc_bits(d.packed);
-    const int type = desc_type(d.packed);
-    const bool inside = e0 >= d.start && e1 <= static_cast<int64_t>(d.start) + d.len;
-    int path = TILE_SLOW;
-    if (inside && type != MODEL_POLY2) path = (b <= 32 && type != MODEL_DIRECT_COPY) ? TILE_NARROW : TILE_WIDE;
-    TileInfo x;
-    x.W = inside ? desc_off(d.packed) + (e0 - d.start) * b : 0;
-    x.p = p;
-    x.info = b | (path << 8);
-    c.tinfo[t] = x;
+    c.tinfo[t] = make_tile_info(c.desc[c.tile_part[t]], e0, e1);
 }
 
 // ---------------------------------------------------------------------------
@@ -475,9 +453,8 @@ __device__ __forceinline__ void pack_wave(const T* __restrict__ v, const Desc* _
                                           uint64_t* __restrict__ s_code) {
     const int b = tile_bits(x.info);
     if (b == 0) return;
-    const Desc d = desc[x.p];
-    const bool direct = desc_type(d.packed) == MODEL_DIRECT_COPY;
-    const int32_t lbase = static_cast<int32_t>(e0 - d.start);
+    const bool direct = tile_type(x.info) == MODEL_DIRECT_COPY;
+    const int32_t lbase = x.lbase;
     const int32_t cnt = static_cast<int32_t>(e1 - e0);
     const uint64_t mask = (b < 64) ? ((1ULL << b) - 1ULL) : ~0ULL;
     const T* __restrict__ vt = v + e0;
@@ -492,7 +469,7 @@ __device__ __forceinline__ void pack_wave(const T* __restrict__ v, const Desc* _
         const int32_t k = r * 64 + lane;
         const uint64_t code =
             direct ? static_cast<uint64_t>(val[r])
-                   : static_cast<uint64_t>(calc_delta<T>(val[r], enc_pred<T>(d.theta0, d.theta1, lbase + k)));
+                   : static_cast<uint64_t>(calc_delta<T>(val[r], enc_pred<T>(x.t0, x.t1, lbase + k)));
         s_code[k] = (k < cnt) ? (code & mask) : 0;
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");

@@ -280,9 +280,8 @@ __device__ __forceinline__ void wave_narrow(const DecArgs& a, const TileInfo& x,
         x0[r] = w[wb + lane];
         xs[r] = w[wb + 64];
     }
-    const Desc d = a.desc[x.p];
-    const int32_t type = desc_type(d.packed);
-    const int32_t lbase = static_cast<int32_t>(e0 - d.start);
+    const int32_t type = tile_type(x.info);
+    const int32_t lbase = x.lbase;
     const uint32_t rel_lane = static_cast<uint32_t>(lane) * static_cast<uint32_t>(b);
 #pragma unroll
     for (int r = 0; r < kWaveRows; ++r) {
@@ -304,7 +303,7 @@ __device__ __forceinline__ void wave_narrow(const DecArgs& a, const TileInfo& x,
             // the builtin returns an unsigned: sign-extend through int32
             delta = static_cast<int32_t>(__builtin_amdgcn_sbfe(static_cast<int>(raw), 0u, static_cast<unsigned>(b)));
         }
-        if (e < e1) emit<T, OP>(a, e, type, d.theta0, d.theta1, lbase + r * 64 + lane, rawv, delta);
+        if (e < e1) emit<T, OP>(a, e, type, x.t0, x.t1, lbase + r * 64 + lane, rawv, delta);
     }
 }
 
@@ -314,9 +313,8 @@ template <class T, int OP>
 __device__ __forceinline__ void wave_wide(const DecArgs& a, const TileInfo& x, int64_t e0, int64_t e1, int lane) {
     const int b = tile_bits(x.info);
     const uint32_t* __restrict__ w = a.words;
-    const Desc d = a.desc[x.p];
-    const int32_t type = desc_type(d.packed);
-    const int32_t lbase = static_cast<int32_t>(e0 - d.start);
+    const int32_t type = tile_type(x.info);
+    const int32_t lbase = x.lbase;
     const uint32_t rel_lane = static_cast<uint32_t>(lane) * static_cast<uint32_t>(b);
     for (int r0 = 0; r0 < kWaveRows; r0 += 4) {
         uint32_t x0[4], x1[4], xs[4];
@@ -347,7 +345,7 @@ __device__ __forceinline__ void wave_wide(const DecArgs& a, const TileInfo& x, i
             const uint64_t rawv = (b >= 64) ? raw : (raw & ((1ULL << b) - 1ULL));
             const int64_t delta =
                 (b >= 64) ? static_cast<int64_t>(raw) : static_cast<int64_t>(raw << (64 - b)) >> (64 - b);
-            if (e < e1) emit<T, OP>(a, e, type, d.theta0, d.theta1, lbase + (r0 + k) * 64 + lane, rawv, delta);
+            if (e < e1) emit<T, OP>(a, e, type, x.t0, x.t1, lbase + (r0 + k) * 64 + lane, rawv, delta);
         }
     }
 }
@@ -361,15 +359,14 @@ __device__ __forceinline__ void wave_plain(const DecArgs& a, const TileInfo& x, 
         const int64_t e = e0 + r * 64 + lane;
         pl[r] = (e < e1) ? a.plain[e] : 0;
     }
-    const Desc d = a.desc[x.p];
-    const int32_t type = desc_type(d.packed);
-    const int32_t lbase = static_cast<int32_t>(e0 - d.start);
+    const int32_t type = tile_type(x.info);
+    const int32_t lbase = x.lbase;
     T* __restrict__ out = static_cast<T*>(a.out);
 #pragma unroll
     for (int r = 0; r < kWaveRows; ++r) {
         const int64_t e = e0 + r * 64 + lane;
         const T v = (type == MODEL_DIRECT_COPY) ? static_cast<T>(pl[r])
-                                                : apply_delta<T>(dec_pred<T>(d.theta0, d.theta1, lbase + r * 64 + lane), pl[r]);
+                                                : apply_delta<T>(dec_pred<T>(x.t0, x.t1, lbase + r * 64 + lane), pl[r]);
         if (e < e1) put(out + e, v);
     }
 }
@@ -424,11 +421,11 @@ template <class T, int OP> __device__ __forceinline__ void wave_tile(const DecAr
     const int path = tile_path(x.info);
     if constexpr (OP == OP_PLAIN) {
         if (path != TILE_SLOW) wave_plain<T>(a, x, e0, e1, lane);
-        else wave_slow<T, OP>(a, e0, e1, x.p, lane);
+        else wave_slow<T, OP>(a, e0, e1, a.tile_part[t], lane);
     } else {
         if (path == TILE_NARROW) wave_narrow<T, OP>(a, x, e0, e1, lane);
         else if (path == TILE_WIDE) wave_wide<T, OP>(a, x, e0, e1, lane);
-        else wave_slow<T, OP>(a, e0, e1, x.p, lane);
+        else wave_slow<T, OP>(a, e0, e1, a.tile_part[t], lane);
     }
 }
 

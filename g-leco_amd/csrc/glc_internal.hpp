@@ -53,17 +53,38 @@ __host__ __device__ inline int64_t desc_off(uint64_t pk) { return static_cast<in
 __host__ __device__ inline int32_t desc_bits(uint64_t pk) { return static_cast<int32_t>((pk >> 48) & 0xFF); }
 __host__ __device__ inline int32_t desc_type(uint64_t pk) { return static_cast<int32_t>(pk >> 56); }
 
-// Per-1024-element tile of the column: bit position of the tile's first
-// element, the partition holding it, and the decode path for the tile.
-struct alignas(16) TileInfo {
-    int64_t W;     // first bit of element t*1024 (valid when the tile is inside partition p)
-    int32_t p;     // last partition with start <= t*1024
-    int32_t info;  // delta_bits | path << 8
+// Per-1024-element tile of the column.  For a tile inside one partition
+// (path NARROW or WIDE) it carries everything a decoder or a lookup needs
+// for any value of the tile in one 32-byte record: the bit position of the
+// tile's first value, the partition's model, and the index of the tile's
+// first value inside its partition.  TILE_SLOW tiles (several partitions,
+// gaps, POLY models) go through tile_part + Desc instead.
+struct alignas(32) TileInfo {
+    int64_t W;      // first bit of element t*1024
+    double t0, t1;  // the partition's theta0, theta1
+    int32_t lbase;  // t*1024 - start of the partition
+    int32_t info;   // delta_bits | path << 8 | model_type << 12
 };
-static_assert(sizeof(TileInfo) == 16, "TileInfo is 16 bytes");
+static_assert(sizeof(TileInfo) == 32, "TileInfo is 32 bytes");
 enum { TILE_SLOW = 0, TILE_NARROW = 1, TILE_WIDE = 2 };
 __host__ __device__ inline int tile_bits(int32_t info) { return info & 0xFF; }
 __host__ __device__ inline int tile_path(int32_t info) { return (info >> 8) & 3; }
+__host__ __device__ inline int tile_type(int32_t info) { return (info >> 12) & 0xF; }
+// The record of tile [e0, e1) whose last partition starting at or before e0 is d.
+__host__ __device__ inline TileInfo make_tile_info(const Desc& d, int64_t e0, int64_t e1) {
+    const int b = desc_bits(d.packed);
+    const int type = desc_type(d.packed);
+    const bool inside = e0 >= d.start && e1 <= static_cast<int64_t>(d.start) + d.len;
+    int path = TILE_SLOW;
+    if (inside && type != MODEL_POLY2) path = (b <= 32 && type != MODEL_DIRECT_COPY) ? TILE_NARROW : TILE_WIDE;
+    TileInfo x;
+    x.W = inside ? desc_off(d.packed) + (e0 - d.start) * b : 0;
+    x.t0 = d.theta0;
+    x.t1 = d.theta1;
+    x.lbase = inside ? static_cast<int32_t>(e0 - d.start) : 0;
+    x.info = b | (path << 8) | ((type & 0xF) << 12);
+    return x;
+}
 constexpr int kSchedBytes = 1024;  // per-XCD work-stealing queues (16 ints apart) + a done counter
 
 // Summary computed on the device when the index is built (one D2H copy).
@@ -121,6 +142,7 @@ struct BatchPlan {
 struct ServeReq {
     const Desc* desc;
     const int32_t* tile_part;
+    const TileInfo* tinfo;
     const uint32_t* words;
     const double* params;
     const int32_t* pos;  // the request's positions (device)

@@ -253,6 +253,7 @@ int launch_step(glc_server* s, Slot& sl, std::vector<Req*>& look) {
         glc::ServeReq q{};
         q.desc = c->d_desc;
         q.tile_part = c->d_tile_part;
+        q.tinfo = c->d_tinfo;
         q.words = c->d_words;
         q.params = c->d_params;
         q.pos = dpos + first;
