@@ -159,6 +159,82 @@ def bench_c4(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_c1(args):
+    """The reference's CPU-runnable case: int32, N = 2^20, v_i = floor(37.5 i) +
+    U[-64, 64], fixed 1024-value partitions; compress, full decompress and
+    10^5 uniform lookups, on the GPU and through the CPU restatement (one
+    thread, as the reference's host code runs, and all cores)."""
+    import numpy as np
+    import torch
+
+    import glc
+    import oracle as O
+    from glc import workloads as W
+
+    n = 1 << 20
+    v = W.c1_int32(n)
+    pos = W.positions_uniform(100_000, n)
+    x = torch.from_numpy(v).cuda()
+    p = torch.from_numpy(pos).cuda()
+
+    def gpu_timed(fn, steps):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+        for a, b in ev:
+            a.record()
+            fn()
+            b.record()
+        torch.cuda.synchronize()
+        return sum(a.elapsed_time(b) for a, b in ev) / steps
+
+    t = time.perf_counter()
+    col = glc.compress(x, glc.PART_FIXED, 1024)
+    torch.cuda.synchronize()
+    comp_ms = (time.perf_counter() - t) * 1e3
+    out = torch.empty_like(x)
+    col.decompress_full(glc.FULL_AUTO, out=out)
+    look = col.random_access(p)
+    torch.cuda.synchronize()
+    assert np.array_equal(out.cpu().numpy(), v) and np.array_equal(look.cpu().numpy(), v[pos])
+    dec_ms = gpu_timed(lambda: col.decompress_full(glc.FULL_AUTO, out=out), args.steps)
+    ra_ms = gpu_timed(lambda: col.random_access(p, out=look), args.steps)
+
+    def cpu_timed(fn, reps=5):
+        ts = []
+        for _ in range(reps):
+            t = time.perf_counter()
+            r = fn()
+            ts.append(time.perf_counter() - t)
+        return statistics.median(ts) * 1e3, r
+
+    cores = O.max_threads()
+    cc_ms, ref = cpu_timed(lambda: O.compress(v, O.PART_FIXED, 1024))
+    cd1_ms, o1 = cpu_timed(lambda: O.decode_full(ref, threads=1))
+    cdn_ms, on = cpu_timed(lambda: O.decode_full(ref, threads=cores))
+    cr1_ms, r1 = cpu_timed(lambda: O.random_access(ref, pos.astype(np.int64), threads=1))
+    assert np.array_equal(o1, v) and np.array_equal(on, v) and np.array_equal(r1, v[pos])
+    line = {
+        "metric": "c1 int32 2^20: full-decompress GB/s and lookups/s, GPU vs the CPU restatement",
+        "value": round(n * 4 / (dec_ms * 1e-3) / 1e9, 2), "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(dec_ms, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "int32", "data": "synthetic (seeded)",
+        "config": {"workload": "c1_int32_N2^20_fixed1024", "n": n, "lookups": int(pos.size),
+                   "compressed_bytes": col.info().compressed_size},
+        "gpu": {"compress_ms": round(comp_ms, 3), "decode_ms": round(dec_ms, 4), "lookups_ms": round(ra_ms, 4),
+                "lookups_M_per_s": round(pos.size / (ra_ms * 1e-3) / 1e6, 1)},
+        "note": "at 4 MiB the GPU decode is launch-bound, not a roofline configuration",
+        "roofline": None,
+        "cpu_baseline": {"value": round(n * 4 / (cd1_ms * 1e-3) / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "port",
+                         "sample": "the whole C1 column: oracle compress, full decode and the 10^5 lookups",
+                         "compress_ms": round(cc_ms, 3), "decode_ms_1thread": round(cd1_ms, 3),
+                         "decode_ms_all_cores": round(cdn_ms, 3), "all_cores": cores,
+                         "lookups_ms_1thread": round(cr1_ms, 3)},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def bench_c3(args):
     """Sorted int64 timestamp-like column, N = 2^29, fixed 4096-value
     partitions: packed against pre-unpacked full decompress and random access
@@ -398,7 +474,7 @@ def main():
     ap.add_argument("--mode", default="worksteal", choices=["worksteal", "bitpacked", "v2", "worksteal_adv"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-log2", type=int, default=24)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5"])
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
     ap.add_argument("--c5-columns", type=int, default=512)
     ap.add_argument("--c5-min-log2", type=int, default=20)
     ap.add_argument("--c5-max-log2", type=int, default=26)
@@ -421,6 +497,8 @@ def main():
         return bench_c4(args)
     if args.workload == "c5":
         return bench_c5(args)
+    if args.workload == "c1":
+        return bench_c1(args)
     if args.workload == "c3":
         if args.log2n == 28:
             args.log2n = 29

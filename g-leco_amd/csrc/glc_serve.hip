@@ -107,6 +107,7 @@ struct Req {
     int64_t id = 0;
     int kind = RK_LOOKUP;
     int32_t col = 0;
+    const glc_column* c = nullptr;  // the resident column (fixed at submit)
     int64_t count = 0;            // lookups or values
     std::vector<int32_t> pos;     // lookup positions (host copy)
     int64_t out_off = -1;         // result block in the pool
@@ -249,7 +250,7 @@ int launch_step(glc_server* s, Slot& sl, std::vector<Req*>& look) {
     int64_t first = 0;
     for (int32_t i = 0; i < R; ++i) {
         Req* r = look[i];
-        const glc_column* c = s->cols[r->col].c;
+        const glc_column* c = r->c;
         glc::ServeReq q{};
         q.desc = c->d_desc;
         q.tile_part = c->d_tile_part;
@@ -279,7 +280,7 @@ int launch_step(glc_server* s, Slot& sl, std::vector<Req*>& look) {
 }
 
 int launch_decompress(glc_server* s, Req* r) {
-    const glc_column* c = s->cols[r->col].c;
+    const glc_column* c = r->c;
     void* out = s->pool + r->out_off;
     int rc = glc::launch_decode_full(c, out, GLC_FULL_AUTO, s->s_dec);
     if (rc) return rc;
@@ -338,12 +339,29 @@ int admit_and_launch(glc_server* s) {
             it = s->pending.erase(it);
         }
     }
-    int rc = GLC_OK;
-    if (!look.empty()) rc = launch_step(s, *slot, look);
-    if (rc) return rc;
-    for (Req* r : dec)
-        if ((rc = launch_decompress(s, r))) return rc;
-    return GLC_OK;
+    // a request whose launch fails is completed with the error right away
+    auto fail_all = [&](const std::vector<Req*>& rs, int rc) {
+        std::lock_guard<std::mutex> lk(s->mu);
+        for (Req* r : rs) complete(s, r, rc, now_s());
+        s->inflight -= static_cast<int32_t>(rs.size());
+        s->cv_done.notify_all();
+    };
+    int first_rc = GLC_OK;
+    if (!look.empty()) {
+        const int rc = launch_step(s, *slot, look);
+        if (rc) {
+            fail_all(look, rc);
+            first_rc = rc;
+        }
+    }
+    for (Req* r : dec) {
+        const int rc = launch_decompress(s, r);
+        if (rc) {
+            fail_all({r}, rc);
+            if (!first_rc) first_rc = rc;
+        }
+    }
+    return first_rc;
 }
 
 void worker_main(glc_server* s) {
@@ -529,6 +547,7 @@ int glc_server_submit_lookup(glc_server* s, int32_t col_id, const int32_t* h_pos
     auto* r = new Req();
     r->kind = RK_LOOKUP;
     r->col = col_id;
+    r->c = c;
     r->count = q;
     r->pos.assign(h_pos, h_pos + q);
     r->out_bytes = q * c->elem_size;
@@ -547,6 +566,7 @@ int glc_server_submit_decompress(glc_server* s, int32_t col_id, int64_t* req_id)
     auto* r = new Req();
     r->kind = RK_DECOMPRESS;
     r->col = col_id;
+    r->c = c;
     r->count = c->n;
     r->out_bytes = c->n * c->elem_size;
     return submit(s, r, req_id);
