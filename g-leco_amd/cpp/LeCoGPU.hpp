@@ -303,21 +303,18 @@ private:
         b.cap = want;
         return b.p;
     }
+    // Host vectors cross through the library's pinned staging ring
+    // (glc_copy_to_device / glc_copy_to_host), in place of the reference's
+    // pageable cudaMemcpy(Async) calls.
     void* upload(int i, const void* h, size_t bytes) {
         void* d = buffer(i, bytes);
-        if (d && bytes && hipMemcpy(d, h, bytes, hipMemcpyHostToDevice) != hipSuccess) {
-            std::cerr << "LeCoGPU: host-to-device copy failed" << std::endl;
-            return nullptr;
-        }
+        if (d && bytes && !ok(glc_copy_to_device(d, h, bytes, nullptr), "host-to-device copy")) return nullptr;
         return d;
     }
     void download(std::vector<T>& out, const void* d, size_t n) {
         out.resize(n);
         (void)hipDeviceSynchronize();
-        if (n && hipMemcpy(out.data(), d, n * sizeof(T), hipMemcpyDeviceToHost) != hipSuccess) {
-            std::cerr << "LeCoGPU: device-to-host copy failed" << std::endl;
-            out.clear();
-        }
+        if (n && !ok(glc_copy_to_host(out.data(), d, n * sizeof(T), nullptr), "device-to-host copy")) out.clear();
     }
 
     CompressedData<T>* wrap(glc_column* col) {

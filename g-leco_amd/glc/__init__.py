@@ -104,6 +104,9 @@ SIGNATURES = [
     ("glc_server_reset_stats", _int, [_vp]),
     ("glc_server_destroy", _int, [_vp]),
     ("glc_batch_free", _int, [_vp]),
+    ("glc_copy_to_host", _int, [_vp, _vp, _sz, _vp]),
+    ("glc_copy_to_device", _int, [_vp, _vp, _sz, _vp]),
+    ("glc_hostio_config", _int, [_vp, _vp, _vp]),
 ]
 
 
@@ -156,6 +159,46 @@ def _devptr(t):
     if not t.is_contiguous():
         raise ValueError("expected a contiguous tensor")
     return C.c_void_p(t.data_ptr())
+
+
+def _hostbuf(a):
+    """(address, bytes) of a C-contiguous host buffer: a numpy array or a CPU torch tensor."""
+    if isinstance(a, np.ndarray):
+        if not a.flags.c_contiguous:
+            raise ValueError("expected a C-contiguous array")
+        return C.c_void_p(a.ctypes.data), a.nbytes
+    if getattr(a, "is_cuda", True):
+        raise ValueError("expected a host buffer (numpy array or CPU tensor)")
+    if not a.is_contiguous():
+        raise ValueError("expected a contiguous tensor")
+    return C.c_void_p(a.data_ptr()), a.numel() * a.element_size()
+
+
+def copy_to_host(dst, src, stream=None):
+    """Copy device tensor src into host buffer dst (same byte size) through
+    the pinned staging ring; returns dst once it holds the data."""
+    h, nb = _hostbuf(dst)
+    if nb != src.numel() * src.element_size():
+        raise ValueError(f"size mismatch: host {nb} B, device {src.numel() * src.element_size()} B")
+    _check(lib().glc_copy_to_host(h, _devptr(src) if nb else None, nb, _stream(stream)))
+    return dst
+
+
+def copy_to_device(dst, src, stream=None):
+    """Copy host buffer src into device tensor dst (same byte size); returns
+    once src may be reused (work queued on the stream afterwards sees dst)."""
+    h, nb = _hostbuf(src)
+    if nb != dst.numel() * dst.element_size():
+        raise ValueError(f"size mismatch: host {nb} B, device {dst.numel() * dst.element_size()} B")
+    _check(lib().glc_copy_to_device(_devptr(dst) if nb else None, h, nb, _stream(stream)))
+    return dst
+
+
+def hostio_config() -> dict:
+    """The current device's staging ring: slots, slot bytes, host copy threads."""
+    s, b, t = C.c_int32(), C.c_int64(), C.c_int32()
+    _check(lib().glc_hostio_config(C.byref(s), C.byref(b), C.byref(t)))
+    return {"slots": s.value, "slot_bytes": b.value, "threads": t.value}
 
 
 def header_checksum(b: bytes) -> int:
@@ -222,6 +265,29 @@ class Column:
             out = torch.empty(n, dtype=_torch_dtype(self.dtype), device="cuda")
         _check(lib().glc_decompress_full(self._h, _devptr(out) if n else None, mode, _stream(stream)))
         return out
+
+    def decompress_full_host(self, mode: int = FULL_AUTO, out=None, scratch=None, stream=None) -> np.ndarray:
+        """Full decompression into host memory (the reference's host-vector
+        decompressFullFile*): decode into `scratch` (a device tensor, allocated
+        when None), then copy through the staging ring into `out`."""
+        n = self.n
+        if out is None:
+            out = np.empty(n, dtype=NP_OF[self.dtype])
+        dev = self.decompress_full(mode, out=scratch, stream=stream)
+        return copy_to_host(out, dev, stream)
+
+    def random_access_host(self, pos: np.ndarray, mode: int = RA_GENERIC, stream=None) -> np.ndarray:
+        """Host positions -> host values (the reference's host-vector decompress,
+        G:615-678): positions and results cross through the staging ring."""
+        import torch
+        pos = np.ascontiguousarray(pos, dtype=np.int32)
+        q = pos.size
+        out = np.empty(q, dtype=NP_OF[self.dtype])
+        if q == 0:
+            return out
+        dpos = torch.empty(q, dtype=torch.int32, device="cuda")
+        copy_to_device(dpos, pos, stream)
+        return copy_to_host(out, self.random_access(dpos, mode, stream=stream), stream)
 
     def decompress_full_fixed(self, partition_size: int, out=None, stream=None):
         import torch

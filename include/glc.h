@@ -180,6 +180,25 @@ int glc_direct_random_access_gpu(const glc_direct_handle* h, const int32_t* d_po
 int glc_direct_random_access_cpu(const glc_direct_handle* h, const int32_t* h_pos, int64_t q, void* h_out);
 int glc_direct_handle_destroy(glc_direct_handle* h);
 
+/* ---- host-memory edges ------------------------------------------------------
+ * The reference copies host vectors with one cudaMemcpy(Async) on pageable
+ * memory: compress input G:375-377; full-decode outputs G:209-210, 261-262,
+ * 339-340, 786, 1781-1782, 1833-1834; lookup positions/results G:650-666,
+ * 728-759, 1296-1328, 1897-1922.  These two calls replace those copies.  A
+ * pageable host buffer goes through a ring of pinned slots held per device
+ * for the life of the process (4 slots of GLC_HOSTIO_SLOT_MIB MiB, default
+ * 32), the copy engine and a pool of GLC_HOSTIO_THREADS host threads
+ * (default min(8, cores)) overlapping; a pinned host buffer takes one DMA.
+ * Both are ordered on `stream` after the work queued before them.
+ * glc_copy_to_host returns when h_dst holds the data.  glc_copy_to_device
+ * returns when h_src may be reused; work queued on `stream` afterwards sees
+ * the data.  Calls on one device are serialised. */
+int glc_copy_to_host(void* h_dst, const void* d_src, size_t bytes, void* stream);
+int glc_copy_to_device(void* d_dst, const void* h_src, size_t bytes, void* stream);
+/* The current device's ring: slot count, slot bytes, host copy threads
+ * (creates the ring on first use). */
+int glc_hostio_config(int32_t* slots, int64_t* slot_bytes, int32_t* threads);
+
 /* ---- batched form (no reference counterpart: many columns, one launch) ----
  * A batch holds the compressed form of many device columns of mixed element
  * types and lengths, each cut into fixed partitions of one size.  Every step

@@ -111,3 +111,16 @@ def test_batch_and_server_argument_checks_need_no_device():
     assert L.glc_server_create(1 << 20, 0, 16, C.byref(h)) == -1     # budget below the staging
     assert L.glc_server_create(1 << 30, 16, 16, C.byref(h)) == -1    # staging too small
     assert not h.value
+
+
+def test_host_copy_argument_checks_need_no_device():
+    L = glc.lib()
+    buf = (C.c_uint8 * 16)()
+    assert L.glc_copy_to_host(None, None, 0, None) == 0          # nothing to copy
+    assert L.glc_copy_to_device(None, None, 0, None) == 0
+    assert L.glc_copy_to_host(None, C.addressof(buf), 16, None) == -1
+    assert L.glc_copy_to_host(C.addressof(buf), None, 16, None) == -1
+    assert L.glc_copy_to_device(None, C.addressof(buf), 16, None) == -1
+    assert L.glc_copy_to_device(C.addressof(buf), None, 16, None) == -1
+    with pytest.raises(ValueError):
+        glc.copy_to_host(np.zeros((4, 4), np.int64)[:, 1], None)     # not contiguous
