@@ -167,6 +167,7 @@ int glc_batch_create(int num_columns, const int32_t* dtypes, const void* const* 
     std::vector<int32_t> part_col, tile_col;
     int64_t Ptot = 0, Ttot = 0, meta = 0, words = 0;
     bool has[4] = {false, false, false, false};
+    bool wg_fit[4] = {false, false, false, false};
     std::vector<glc::ArenaLayout> lay(num_columns);
     b->meta_off.resize(num_columns);
     b->words_off.resize(num_columns);
@@ -198,6 +199,7 @@ int glc_batch_create(int num_columns, const int32_t* dtypes, const void* const* 
         c.tile_base = Ttot;
         c.dtype = dt;
         c.fit_bs = glc::fit_block_size(std::min<int64_t>(partition_size, ni));
+        if (P > 0 && c.fit_bs != 256) wg_fit[dt] = true;
         c.words_reserved = b->words_res[i];
         part_col.insert(part_col.end(), static_cast<size_t>(P), i);
         tile_col.insert(tile_col.end(), static_cast<size_t>(T), i);
@@ -260,7 +262,10 @@ int glc_batch_create(int num_columns, const int32_t* dtypes, const void* const* 
     p.Ttot = Ttot;
     p.C = num_columns;
     p.S = partition_size;
-    for (int d = 0; d < 4; ++d) p.has_dtype[d] = has[d];
+    for (int d = 0; d < 4; ++d) {
+        p.has_dtype[d] = has[d];
+        p.has_wg_fit[d] = wg_fit[d];
+    }
     if (hipHostMalloc(reinterpret_cast<void**>(&b->h_outs), sizeof(void*) * std::max(num_columns, 1)) != hipSuccess)
         return bail(fail(GLC_ERR_OOM, "pinned allocation of the output table failed"));
     if (hipEventCreateWithFlags(&b->ev_outs, hipEventDisableTiming) != hipSuccess ||

@@ -36,6 +36,33 @@ struct FitSmem {
     int64_t mx[kFitThreads / 64];
 };
 
+// theta from the four sums (:140-160): determinant test, then slope and
+// intercept, each with one fma.
+__device__ __forceinline__ void fit_theta(int32_t n, double Sx, double Sy, double Sxx, double Sxy, double& t0,
+                                          double& t1) {
+    const double dn = static_cast<double>(n);
+    const double det = __builtin_fma(dn, Sxx, -(Sx * Sx));
+    if (__builtin_fabs(det) > 1e-10) {
+        t1 = __builtin_fma(dn, Sxy, -(Sx * Sy)) / det;
+        t0 = __builtin_fma(-t1, Sx, Sy) / dn;
+    } else {
+        t1 = 0.0;
+        t0 = Sy / dn;
+    }
+}
+
+template <class T> __device__ __forceinline__ int64_t abs_delta(T val, double t0, double t1, int32_t i) {
+    const int64_t dl = calc_delta<T>(val, enc_pred<T>(t0, t1, i));
+    // -LLONG_MIN stays negative and is ignored by the max, as in :171-174
+    return dl < 0 ? static_cast<int64_t>(0ULL - static_cast<uint64_t>(dl)) : dl;
+}
+
+__device__ __forceinline__ int32_t bits_for(int64_t m) {
+    if (m <= 0) return 0;
+    const int32_t bb = 64 - __builtin_clzll(static_cast<unsigned long long>(m)) + 1;
+    return bb > 64 ? 64 : bb;
+}
+
 // Partition p of n values at x (n > 0).  Called by the whole workgroup.
 template <class T>
 __device__ __forceinline__ void fit_one(const T* __restrict__ x, int32_t n, int bs, int64_t p, int32_t* __restrict__ type,
@@ -78,17 +105,8 @@ __device__ __forceinline__ void fit_one(const T* __restrict__ x, int32_t n, int 
         __syncthreads();
     }
     if (tid == 0) {
-        const double dn = static_cast<double>(n);
-        const double Sx = sm.sum[0][0], Sy = sm.sum[1][0], Sxx = sm.sum[2][0], Sxy = sm.sum[3][0];
-        const double det = __builtin_fma(dn, Sxx, -(Sx * Sx));
         double t0, t1;
-        if (__builtin_fabs(det) > 1e-10) {
-            t1 = __builtin_fma(dn, Sxy, -(Sx * Sy)) / det;
-            t0 = __builtin_fma(-t1, Sx, Sy) / dn;
-        } else {
-            t1 = 0.0;
-            t0 = Sy / dn;
-        }
+        fit_theta(n, sm.sum[0][0], sm.sum[1][0], sm.sum[2][0], sm.sum[3][0], t0, t1);
         sm.theta[0] = t0; sm.theta[1] = t1;
         type[p] = MODEL_LINEAR;
         params[4 * p] = t0; params[4 * p + 1] = t1; params[4 * p + 2] = 0.0; params[4 * p + 3] = 0.0;
@@ -97,9 +115,8 @@ __device__ __forceinline__ void fit_one(const T* __restrict__ x, int32_t n, int 
     const double t0 = sm.theta[0], t1 = sm.theta[1];
     int64_t mx = 0;
     for (int32_t i = tid; i < n; i += kFitThreads) {
-        const int64_t dl = calc_delta<T>(x[i], enc_pred<T>(t0, t1, i));
-        const int64_t ab = dl < 0 ? static_cast<int64_t>(0ULL - static_cast<uint64_t>(dl)) : dl;
-        mx = ab > mx ? ab : mx;  // -LLONG_MIN stays negative and is ignored, as in :171-174
+        const int64_t ab = abs_delta<T>(x[i], t0, t1, i);
+        mx = ab > mx ? ab : mx;
     }
     for (int o = 32; o > 0; o >>= 1) {
         const int64_t y = __shfl_xor(mx, o, 64);
@@ -111,14 +128,132 @@ __device__ __forceinline__ void fit_one(const T* __restrict__ x, int32_t n, int 
         int64_t m = sm.mx[0];
         for (int w = 1; w < kFitThreads / 64; ++w) m = sm.mx[w] > m ? sm.mx[w] : m;
         err[p] = m;
-        int32_t bb = 0;
-        if (m > 0) {
-            bb = 64 - __builtin_clzll(static_cast<unsigned long long>(m)) + 1;
-            bb = bb > 64 ? 64 : bb;
-        }
-        bits[p] = bb;
+        bits[p] = bits_for(m);
     }
     __syncthreads();
+}
+
+// One wave per partition, for the block size 256 (every partition of 256 or
+// more values at the reference's launch, G:519-520).  Lane l plays the
+// reference's threads l, l+64, l+128 and l+192 (sets j = 0..3), each with its
+// own partial sums over i = tid, tid+256, ... in increasing i.  The tree
+// s = 128, 64, 32, ..., 1 then adds the same operands in the same order: the
+// steps 128 and 64 add sets inside a lane (sum[t] += sum[t + s] with t + s in
+// set j + s/64 of the same lane), the steps 32..1 are shuffles.  No LDS and no
+// barrier: a workgroup's waves work on different partitions.
+constexpr int kFitWaves = 4;
+constexpr int kFitWaveBs = 256;
+
+// Partition p of n > 0 values at x, fitted by one wave.
+template <class T>
+__device__ __forceinline__ void fit_wave(const T* __restrict__ x, int32_t n, int64_t p, int32_t* __restrict__ type,
+                                         double* __restrict__ params, int32_t* __restrict__ bits,
+                                         int64_t* __restrict__ err, int lane) {
+    double a[4] = {0.0, 0.0, 0.0, 0.0}, b[4] = {0.0, 0.0, 0.0, 0.0};
+    double c[4] = {0.0, 0.0, 0.0, 0.0}, d[4] = {0.0, 0.0, 0.0, 0.0};
+    int ovf = 0;
+    // 512 values per step: element base + 64k + lane belongs to set k & 3,
+    // and within a set the steps visit i in increasing order.  Loads are
+    // branch-free (index clamped to n - 1) so all eight are in flight at once.
+    for (int32_t base = 0; base < n; base += 2 * kFitWaveBs) {
+        T y[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int32_t i = base + 64 * k + lane;
+            y[k] = x[i < n ? i : n - 1];
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int32_t i = base + 64 * k + lane;
+            if (i < n) {
+                ovf |= might_overflow_double<T>(y[k]) ? 1 : 0;
+                const double xi = static_cast<double>(i);
+                const double yi = static_cast<double>(y[k]);
+                a[k & 3] = a[k & 3] + xi;
+                b[k & 3] = b[k & 3] + yi;
+                c[k & 3] = __builtin_fma(xi, xi, c[k & 3]);
+                d[k & 3] = __builtin_fma(xi, yi, d[k & 3]);
+            }
+        }
+    }
+    if (__any(ovf)) {  // :95-106 DIRECT_COPY
+        if (lane == 0) {
+            type[p] = MODEL_DIRECT_COPY;
+            params[4 * p] = 0.0; params[4 * p + 1] = 0.0; params[4 * p + 2] = 0.0; params[4 * p + 3] = 0.0;
+            err[p] = 0;
+            bits[p] = static_cast<int32_t>(sizeof(T) * 8);
+        }
+        return;
+    }
+    // s = 128: t < 128 adds t + 128 (set j + 2); s = 64: t < 64 adds t + 64 (set 1)
+    double Sx = (a[0] + a[2]) + (a[1] + a[3]);
+    double Sy = (b[0] + b[2]) + (b[1] + b[3]);
+    double Sxx = (c[0] + c[2]) + (c[1] + c[3]);
+    double Sxy = (d[0] + d[2]) + (d[1] + d[3]);
+    for (int h = 32; h > 0; h >>= 1) {
+        Sx = Sx + __shfl_down(Sx, h, 64);
+        Sy = Sy + __shfl_down(Sy, h, 64);
+        Sxx = Sxx + __shfl_down(Sxx, h, 64);
+        Sxy = Sxy + __shfl_down(Sxy, h, 64);
+    }
+    // lane 0 holds the sums; every lane computes the same theta from them
+    Sx = __shfl(Sx, 0, 64);
+    Sy = __shfl(Sy, 0, 64);
+    Sxx = __shfl(Sxx, 0, 64);
+    Sxy = __shfl(Sxy, 0, 64);
+    double t0, t1;
+    fit_theta(n, Sx, Sy, Sxx, Sxy, t0, t1);
+    int64_t mx = 0;
+    const bool fast = enc_fast_range<T>(__builtin_fma(t1, 0.0, t0), __builtin_fma(t1, static_cast<double>(n - 1), t0));
+    for (int32_t base = 0; base < n; base += 2 * kFitWaveBs) {
+        T y[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int32_t i = base + 64 * k + lane;
+            y[k] = x[i < n ? i : n - 1];
+        }
+        // the clamped index repeats element n - 1: same value
+        if (fast) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const int32_t i = base + 64 * k + lane;
+                const int64_t dl = enc_delta_fast<T>(y[k], t0, t1, i < n ? i : n - 1);
+                const int64_t ab = dl < 0 ? static_cast<int64_t>(0ULL - static_cast<uint64_t>(dl)) : dl;
+                mx = ab > mx ? ab : mx;
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const int32_t i = base + 64 * k + lane;
+                const int64_t ab = abs_delta<T>(y[k], t0, t1, i < n ? i : n - 1);
+                mx = ab > mx ? ab : mx;
+            }
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const int64_t y = __shfl_xor(mx, o, 64);
+        mx = y > mx ? y : mx;
+    }
+    if (lane == 0) {
+        type[p] = MODEL_LINEAR;
+        params[4 * p] = t0; params[4 * p + 1] = t1; params[4 * p + 2] = 0.0; params[4 * p + 3] = 0.0;
+        err[p] = mx;
+        bits[p] = bits_for(mx);
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(64 * kFitWaves) void k_fit_wave(const T* __restrict__ v, const int32_t* __restrict__ st,
+                                                            const int32_t* __restrict__ en, int32_t* __restrict__ type,
+                                                            double* __restrict__ params, int32_t* __restrict__ bits,
+                                                            int64_t* __restrict__ err, int64_t P) {
+    const int lane = threadIdx.x & 63;
+    const int64_t p = static_cast<int64_t>(blockIdx.x) * kFitWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (p >= P) return;
+    const int32_t s0 = st[p];
+    const int32_t n = en[p] - s0;
+    if (n <= 0) return;  // :62, entry keeps its initial values
+    fit_wave<T>(v + s0, n, p, type, params, bits, err, lane);
 }
 
 template <class T>
@@ -144,13 +279,30 @@ __global__ __launch_bounds__(kFitThreads) void k_fit_batch(const BatchCol* __res
     __shared__ FitSmem sm;
     for (int64_t g = blockIdx.x; g < Ptot; g += gridDim.x) {
         const BatchCol& c = cols[part_col[g]];
-        if (c.dtype != dtype) continue;
+        if (c.dtype != dtype || c.fit_bs == kFitWaveBs) continue;  // block size 256: k_fit_wave_batch
         const int64_t p = g - c.part_base;
         const int32_t s0 = c.start[p];
         const int32_t n = c.end[p] - s0;
         if (n <= 0) continue;
         fit_one<T>(static_cast<const T*>(c.values) + s0, n, c.fit_bs, p, c.type, c.params, c.bits, c.err, sm);
     }
+}
+
+// Batched fit, one wave per partition of the columns whose block size is 256.
+template <class T>
+__global__ __launch_bounds__(64 * kFitWaves) void k_fit_wave_batch(const BatchCol* __restrict__ cols,
+                                                                  const int32_t* __restrict__ part_col, int64_t Ptot,
+                                                                  int dtype) {
+    const int lane = threadIdx.x & 63;
+    const int64_t g = static_cast<int64_t>(blockIdx.x) * kFitWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (g >= Ptot) return;
+    const BatchCol& c = cols[part_col[g]];
+    if (c.dtype != dtype || c.fit_bs != kFitWaveBs) return;
+    const int64_t p = g - c.part_base;
+    const int32_t s0 = c.start[p];
+    const int32_t n = c.end[p] - s0;
+    if (n <= 0) return;
+    fit_wave<T>(static_cast<const T*>(c.values) + s0, n, p, c.type, c.params, c.bits, c.err, lane);
 }
 
 // ---------------------------------------------------------------------------
@@ -443,14 +595,22 @@ __device__ __forceinline__ void pack_tile(const T* __restrict__ v, const Desc* _
 // wave's LDS slice.  Phase 2, word-major: lane j assembles output words j,
 // j+64, ... from the (at most 32/bits + 2) codes that overlap each word, so
 // no word is shared between lanes and no atomics are needed inside the
-// tile.  The tile's first and last words may be shared with the
-// neighbouring tiles: those two are OR-ed into the zero-initialised payload
-// atomically; every other word is stored whole.
+// tile.  Every word the tile covers completely is stored whole.  A word the
+// tile starts or ends inside is shared with the neighbouring tile; its
+// partial value is returned as an edge (wave-uniform) for merge_edges.
 constexpr int kPackWaves = 4;
+
+struct PackEdge {
+    uint32_t* p;  // nullptr: no partial word
+    uint32_t v;
+};
+
 template <class T>
 __device__ __forceinline__ void pack_wave(const T* __restrict__ v, const Desc* __restrict__ desc, const TileInfo& x,
                                           int64_t e0, int64_t e1, uint32_t* __restrict__ words, int lane,
-                                          uint64_t* __restrict__ s_code) {
+                                          uint64_t* __restrict__ s_code, PackEdge& ef, PackEdge& el) {
+    ef = PackEdge{nullptr, 0};
+    el = PackEdge{nullptr, 0};
     const int b = tile_bits(x.info);
     if (b == 0) return;
     const bool direct = tile_type(x.info) == MODEL_DIRECT_COPY;
@@ -464,13 +624,25 @@ __device__ __forceinline__ void pack_wave(const T* __restrict__ v, const Desc* _
         const int32_t k = r * 64 + lane;
         val[r] = vt[k < cnt ? k : cnt - 1];
     }
+    // Tiles whose predictions need no saturation (the common case) take the
+    // branch-free residual; the codes are the same either way.
+    if (!direct && enc_fast_range<T>(__builtin_fma(x.t1, static_cast<double>(lbase), x.t0),
+                                     __builtin_fma(x.t1, static_cast<double>(lbase + cnt - 1), x.t0))) {
 #pragma unroll
-    for (int r = 0; r < kIndexTile / 64; ++r) {
-        const int32_t k = r * 64 + lane;
-        const uint64_t code =
-            direct ? static_cast<uint64_t>(val[r])
-                   : static_cast<uint64_t>(calc_delta<T>(val[r], enc_pred<T>(x.t0, x.t1, lbase + k)));
-        s_code[k] = (k < cnt) ? (code & mask) : 0;
+        for (int r = 0; r < kIndexTile / 64; ++r) {
+            const int32_t k = r * 64 + lane;
+            const uint64_t code = static_cast<uint64_t>(enc_delta_fast<T>(val[r], x.t0, x.t1, lbase + k));
+            s_code[k] = (k < cnt) ? (code & mask) : 0;
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < kIndexTile / 64; ++r) {
+            const int32_t k = r * 64 + lane;
+            const uint64_t code =
+                direct ? static_cast<uint64_t>(val[r])
+                       : static_cast<uint64_t>(calc_delta<T>(val[r], enc_pred<T>(x.t0, x.t1, lbase + k)));
+            s_code[k] = (k < cnt) ? (code & mask) : 0;
+        }
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -478,10 +650,19 @@ __device__ __forceinline__ void pack_wave(const T* __restrict__ v, const Desc* _
     const int64_t w0 = x.W >> 5;
     const int32_t r0 = static_cast<int32_t>(x.W & 31);
     const int32_t Wt = (r0 + cnt * b + 31) >> 5;  // <= 2049
+    // floor(y / b) as mul_hi(y, ceil(2^32 / b)): exact for y * b < 2^32, and
+    // here y <= 32 * 2049 + 31 < 2^17 with b <= 64.
+    const uint32_t rcp = 0xFFFFFFFFu / static_cast<uint32_t>(b) + 1u;
+    const auto div_b = [&](int32_t y) -> int32_t {
+        return b == 1 ? y : static_cast<int32_t>(__umulhi(static_cast<uint32_t>(y), rcp));
+    };
+    const bool start_partial = r0 != 0;
+    const bool end_partial = ((r0 + cnt * b) & 31) != 0;
+    uint32_t vf = 0, vl = 0;
     for (int32_t j = lane; j < Wt; j += 64) {
         const int32_t lo_bit = 32 * j - r0;  // first stream bit of word j (relative to element e0)
-        const int32_t k0 = lo_bit < 0 ? 0 : lo_bit / b;
-        int32_t k1 = (lo_bit + 31) / b;
+        const int32_t k0 = lo_bit < 0 ? 0 : div_b(lo_bit);
+        int32_t k1 = div_b(lo_bit + 31);
         if (k1 > cnt - 1) k1 = cnt - 1;
         uint32_t wv = 0;
         for (int32_t k = k0; k <= k1; ++k) {
@@ -489,12 +670,53 @@ __device__ __forceinline__ void pack_wave(const T* __restrict__ v, const Desc* _
             const int32_t pos = k * b - lo_bit;  // in (-b, 32)
             wv |= (pos >= 0) ? static_cast<uint32_t>(code << pos) : static_cast<uint32_t>(code >> (-pos));
         }
-        uint32_t* p = words + w0 + j;
-        if (j == 0 || j == Wt - 1) {
-            if (wv) atomicOr(p, wv);
-        } else {
-            *p = wv;
+        if (j == 0 && start_partial) vf = wv;
+        else if (j == Wt - 1 && end_partial) vl = wv;
+        else words[w0 + j] = wv;
+    }
+    // word 0 is lane 0's first word; word Wt - 1 is lane (Wt - 1) % 64's last
+    if (start_partial) ef = PackEdge{words + w0, static_cast<uint32_t>(__builtin_amdgcn_readlane(vf, 0))};
+    if (end_partial && !(start_partial && Wt == 1))
+        el = PackEdge{words + w0 + Wt - 1, static_cast<uint32_t>(__builtin_amdgcn_readlane(vl, (Wt - 1) & 63))};
+}
+
+// The partial words of a workgroup's tiles, in bit order (first and last
+// edge of each wave).  Two entries with the same word can only be adjacent
+// (non-empty tiles of 1024 values span at least 32 bits); such a pair is the
+// word's whole content and is stored plainly.  The first and last entries
+// may share their word with a tile of another workgroup: OR-ed atomically.
+// Any other unpaired entry shares its word only with a tile of the
+// partition-spanning path, whose own kernel runs later and ORs its part in
+// atomically, so a plain store is correct there too.  One thread.
+__device__ __forceinline__ void merge_edges(const PackEdge* __restrict__ e, int m) {
+    int first = -1, last = -1;
+    for (int k = 0; k < m; ++k)
+        if (e[k].p) {
+            if (first < 0) first = k;
+            last = k;
         }
+    if (first < 0) return;
+    int k = first;
+    while (k <= last) {
+        uint32_t* p = e[k].p;
+        uint32_t v = e[k].v;
+        bool outer = k == first;
+        int k2 = k + 1;
+        while (k2 <= last && !e[k2].p) ++k2;
+        if (k2 <= last && e[k2].p == p) {
+            v |= e[k2].v;
+            outer = outer || k2 == last;
+            k2 += 1;
+            while (k2 <= last && !e[k2].p) ++k2;
+        } else {
+            outer = outer || k == last;
+        }
+        if (outer) {
+            if (v) atomicOr(p, v);
+        } else {
+            *p = v;
+        }
+        k = k2;
     }
 }
 
@@ -503,15 +725,25 @@ __global__ __launch_bounds__(64 * kPackWaves) void k_pack_wave(const T* __restri
                                                                const TileInfo* __restrict__ tinfo, int64_t n,
                                                                uint32_t* __restrict__ words) {
     __shared__ uint64_t s_code[kPackWaves][kIndexTile];
+    __shared__ PackEdge s_edge[2 * kPackWaves];
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t t = static_cast<int64_t>(blockIdx.x) * kPackWaves + wave;
     const int64_t e0 = t * kIndexTile;
-    if (e0 >= n) return;
-    const TileInfo x = tinfo[t];
-    if (tile_path(x.info) == TILE_SLOW) return;
-    const int64_t e1 = (e0 + kIndexTile < n) ? e0 + kIndexTile : n;
-    pack_wave<T>(v, desc, x, e0, e1, words, lane, s_code[wave]);
+    PackEdge ef{nullptr, 0}, el{nullptr, 0};
+    if (e0 < n) {
+        const TileInfo x = tinfo[t];
+        if (tile_path(x.info) != TILE_SLOW) {
+            const int64_t e1 = (e0 + kIndexTile < n) ? e0 + kIndexTile : n;
+            pack_wave<T>(v, desc, x, e0, e1, words, lane, s_code[wave], ef, el);
+        }
+    }
+    if (lane == 0) {
+        s_edge[2 * wave] = ef;
+        s_edge[2 * wave + 1] = el;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) merge_edges(s_edge, 2 * kPackWaves);
 }
 
 // The tiles the wave pack leaves (several partitions, gaps, POLY models):
@@ -539,18 +771,31 @@ __global__ __launch_bounds__(64 * kPackWaves) void k_pack_wave_batch(const Batch
                                                                      const int32_t* __restrict__ tile_col, int64_t Ttot,
                                                                      int dtype) {
     __shared__ uint64_t s_code[kPackWaves][kIndexTile];
+    __shared__ PackEdge s_edge[2 * kPackWaves];
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t g = static_cast<int64_t>(blockIdx.x) * kPackWaves + wave;
-    if (g >= Ttot) return;
-    const BatchCol& c = cols[tile_col[g]];
-    if (c.dtype != dtype || c.payload_words > c.words_reserved) return;
-    const int64_t t = g - c.tile_base;
-    const TileInfo x = c.tinfo[t];
-    if (tile_path(x.info) == TILE_SLOW) return;
-    const int64_t e0 = t * kIndexTile;
-    const int64_t e1 = (e0 + kIndexTile < c.n) ? e0 + kIndexTile : c.n;
-    pack_wave<T>(static_cast<const T*>(c.values), c.desc, x, e0, e1, c.words, lane, s_code[wave]);
+    PackEdge ef{nullptr, 0}, el{nullptr, 0};
+    if (g < Ttot) {
+        const BatchCol& c = cols[tile_col[g]];
+        if (c.dtype == dtype && c.payload_words <= c.words_reserved) {
+            const int64_t t = g - c.tile_base;
+            const TileInfo x = c.tinfo[t];
+            if (tile_path(x.info) != TILE_SLOW) {
+                const int64_t e0 = t * kIndexTile;
+                const int64_t e1 = (e0 + kIndexTile < c.n) ? e0 + kIndexTile : c.n;
+                pack_wave<T>(static_cast<const T*>(c.values), c.desc, x, e0, e1, c.words, lane, s_code[wave], ef, el);
+            }
+        }
+    }
+    // tiles of different columns never share a word (separate payloads), so
+    // their edges never pair
+    if (lane == 0) {
+        s_edge[2 * wave] = ef;
+        s_edge[2 * wave + 1] = el;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) merge_edges(s_edge, 2 * kPackWaves);
 }
 
 template <class T>
@@ -724,6 +969,13 @@ int launch_fit(int dtype, const void* d_values, glc_column* c, int32_t bs, hipSt
     if (bs < 32 || bs > kFitThreads) return fail(GLC_ERR_INVALID_ARG, "fit block size out of range");
     return dispatch_dtype(dtype, [&](auto t) {
         using T = decltype(t);
+        if (bs == kFitWaveBs) {
+            const unsigned g = static_cast<unsigned>((c->P + kFitWaves - 1) / kFitWaves);
+            hipLaunchKernelGGL(k_fit_wave<T>, dim3(g), dim3(64 * kFitWaves), 0, s, static_cast<const T*>(d_values),
+                               c->d_start, c->d_end, c->d_type, c->d_params, c->d_bits, c->d_err, c->P);
+            GLC_HIP(hipGetLastError());
+            return GLC_OK;
+        }
         const unsigned g = static_cast<unsigned>(c->P < (1LL << 30) ? c->P : (1LL << 30));
         hipLaunchKernelGGL(k_fit<T>, dim3(g), dim3(kFitThreads), 0, s, static_cast<const T*>(d_values), c->d_start,
                            c->d_end, c->d_type, c->d_params, c->d_bits, c->d_err, c->P, bs);
@@ -869,8 +1121,13 @@ int launch_batch_compress(const BatchPlan& b, hipStream_t s) {
         if (!b.has_dtype[dt]) continue;
         int rc = dispatch_dtype(dt, [&](auto t) {
             using T = decltype(t);
-            hipLaunchKernelGGL(k_fit_batch<T>, dim3(gf), dim3(kFitThreads), 0, s, b.cols, b.part_col, b.Ptot, dt);
+            hipLaunchKernelGGL(k_fit_wave_batch<T>, dim3(static_cast<unsigned>((b.Ptot + kFitWaves - 1) / kFitWaves)),
+                               dim3(64 * kFitWaves), 0, s, b.cols, b.part_col, b.Ptot, dt);
             GLC_HIP(hipGetLastError());
+            if (b.has_wg_fit[dt]) {
+                hipLaunchKernelGGL(k_fit_batch<T>, dim3(gf), dim3(kFitThreads), 0, s, b.cols, b.part_col, b.Ptot, dt);
+                GLC_HIP(hipGetLastError());
+            }
             return GLC_OK;
         });
         if (rc) return rc;

@@ -69,6 +69,37 @@ template <class T> __device__ __forceinline__ int64_t calc_delta(T actual, T pre
     }
 }
 
+// Encoder residual without saturation branches, for predictions known to lie
+// in enc_fast_range.  The prediction fma(theta1, i, theta0) is monotonic in
+// i (the exact value is, and rounding preserves order), and so are round()
+// and the conversion, so checking the two end indices of a run of indices
+// covers every index between them.  Inside the range:
+//   64-bit T: round(x) is an integer of magnitude <= 2^50, converted exactly
+//             by the 1.5 * 2^52 constant (sat_cast's own fast path);
+//   32-bit T: round(x) lies inside T, converted exactly by one cvt;
+//   uint64:   p <= 2^50, so a - p wraps only when a >= 2^63, and then
+//             calc_delta saturates to INT64_MAX (its a >= p branch).
+// Results equal calc_delta<T>(a, enc_pred<T>(t0, t1, i)) bit for bit.
+template <class T> __device__ __forceinline__ bool enc_fast_range(double x_lo, double x_hi) {
+    const double lo = __builtin_fmin(x_lo, x_hi), hi = __builtin_fmax(x_lo, x_hi);
+    if constexpr (std::is_same<T, int32_t>::value) return lo >= -2147483647.0 && hi <= 2147483646.0;
+    else if constexpr (std::is_same<T, uint32_t>::value) return lo >= 0.0 && hi <= 4294967294.0;
+    else if constexpr (std::is_same<T, int64_t>::value) return lo >= -1125899906842624.0 && hi <= 1125899906842624.0;
+    else return lo >= 0.0 && hi <= 1125899906842624.0;
+}
+
+template <class T> __device__ __forceinline__ int64_t enc_delta_fast(T a, double t0, double t1, int32_t i) {
+    const double r = __builtin_round(__builtin_fma(t1, static_cast<double>(i), t0));
+    if constexpr (sizeof(T) == 4) {
+        return static_cast<int64_t>(a) - static_cast<int64_t>(static_cast<T>(r));
+    } else {
+        const uint64_t p = static_cast<uint64_t>(__double_as_longlong(r + 6755399441055744.0) - 0x4338000000000000LL);
+        const int64_t s = static_cast<int64_t>(static_cast<uint64_t>(a) - p);
+        if constexpr (std::is_signed<T>::value) return s;
+        else return (static_cast<int64_t>(a) < 0 && s < 0) ? INT64_MAX : s;
+    }
+}
+
 // core/MathHelpers.cuh:55-84
 template <class T> __device__ __forceinline__ T apply_delta(T pred, int64_t d) {
     typedef typename std::make_unsigned<T>::type U;

@@ -137,6 +137,7 @@ struct BatchPlan {
     int32_t C;
     int32_t S;
     bool has_dtype[4];
+    bool has_wg_fit[4];  // a column of this type has a fit block size other than 256
 };
 // One lookup request of a serving step (device resident, built per step).
 struct ServeReq {
