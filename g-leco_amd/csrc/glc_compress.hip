@@ -424,40 +424,69 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_batch(const BatchCol* __r
 // ---------------------------------------------------------------------------
 // Decode index: Desc[P] + tile_part[] + validation summary.
 // ---------------------------------------------------------------------------
+// Wave-wide max of a value every lane holds (no lane has exited).
+template <class V> __device__ __forceinline__ V wave_max_all(V x) {
+    for (int o = 32; o > 0; o >>= 1) {
+        const V y = __shfl_xor(x, o, 64);
+        x = y > x ? y : x;
+    }
+    return x;
+}
+
+// One thread per partition.  The column summary (flags, max bits, total
+// bits) is reduced across the wave first, so each wave issues one atomic
+// per field instead of one per partition.
 __global__ void k_build_index(const int32_t* __restrict__ st, const int32_t* __restrict__ en,
                               const int32_t* __restrict__ type, const double* __restrict__ params,
                               const int32_t* __restrict__ bits, const int64_t* __restrict__ off, int64_t P,
                               int64_t n, Desc* __restrict__ desc, int32_t* __restrict__ tile_part, int64_t n_index,
                               IndexSummary* __restrict__ sum) {
     const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (p >= P) return;
-    const int32_t s = st[p], e = en[p], b = bits[p], ty = type[p];
-    const int32_t len = e > s ? e - s : 0;
-    Desc d;
-    d.theta0 = params[4 * p];
-    d.theta1 = params[4 * p + 1];
-    d.packed = desc_pack(off[p], b, ty);
-    d.start = s;
-    d.len = len;
-    desc[p] = d;
-    // validation (sorted, non-overlapping, inside [0, n))
-    int bad = (s < 0 || e < s || static_cast<int64_t>(e) > n) ? 1 : 0;
-    const int32_t s_next = (p + 1 < P) ? st[p + 1] : 0x7FFFFFFF;
-    if (p + 1 < P && e > s_next) bad = 1;
-    if (p + 1 < P && s_next < s) bad = 1;
-    if (bad) atomicOr(&sum->malformed, 1);
-    if (b < 0 || b > 64) atomicOr(&sum->bad_bits, 1);
-    if (ty == MODEL_POLY2) atomicOr(&sum->has_poly, 1);
-    atomicMax(&sum->max_bits, b);
-    const int64_t endbit = off[p] + static_cast<int64_t>(len) * b;
-    atomicMax(reinterpret_cast<unsigned long long*>(&sum->total_bits), static_cast<unsigned long long>(endbit));
-    // fixed-size detection against S = len of partition 0
-    const int64_t S = static_cast<int64_t>(en[0]) - st[0];
-    const int64_t fs = S * p;
-    const int64_t fe = (fs + S < n) ? fs + S : n;
-    if (S <= 0 || s != fs || e != fe || (p == P - 1 && e != n)) atomicOr(&sum->not_fixed, 1);
+    const bool live = p < P;  // every lane stays for the wave reductions
+    int bad = 0, bad_bits = 0, poly = 0, not_fixed = 0;
+    int32_t b = 0;
+    int64_t endbit = 0;
+    int32_t s = 0, s_next = 0x7FFFFFFF;
+    if (live) {
+        const int32_t e = en[p], ty = type[p];
+        s = st[p];
+        b = bits[p];
+        const int32_t len = e > s ? e - s : 0;
+        Desc d;
+        d.theta0 = params[4 * p];
+        d.theta1 = params[4 * p + 1];
+        d.packed = desc_pack(off[p], b, ty);
+        d.start = s;
+        d.len = len;
+        desc[p] = d;
+        // validation (sorted, non-overlapping, inside [0, n))
+        bad = (s < 0 || e < s || static_cast<int64_t>(e) > n) ? 1 : 0;
+        if (p + 1 < P) s_next = st[p + 1];
+        if (p + 1 < P && e > s_next) bad = 1;
+        if (p + 1 < P && s_next < s) bad = 1;
+        bad_bits = (b < 0 || b > 64) ? 1 : 0;
+        poly = ty == MODEL_POLY2 ? 1 : 0;
+        endbit = off[p] + static_cast<int64_t>(len) * b;
+        // fixed-size detection against S = len of partition 0
+        const int64_t S = static_cast<int64_t>(en[0]) - st[0];
+        const int64_t fs = S * p;
+        const int64_t fe = (fs + S < n) ? fs + S : n;
+        not_fixed = (S <= 0 || s != fs || e != fe || (p == P - 1 && e != n)) ? 1 : 0;
+    }
+    const bool any_bad = __any(bad), any_bad_bits = __any(bad_bits), any_poly = __any(poly);
+    const bool any_not_fixed = __any(not_fixed);
+    const int32_t wmax_bits = wave_max_all(live ? b : INT32_MIN);
+    const uint64_t wend = wave_max_all(live ? static_cast<uint64_t>(endbit) : 0ULL);  // unsigned, as the atomic
+    if ((threadIdx.x & 63) == 0 && __any(live)) {
+        if (any_bad) atomicOr(&sum->malformed, 1);
+        if (any_bad_bits) atomicOr(&sum->bad_bits, 1);
+        if (any_poly) atomicOr(&sum->has_poly, 1);
+        if (any_not_fixed) atomicOr(&sum->not_fixed, 1);
+        atomicMax(&sum->max_bits, wmax_bits);
+        atomicMax(reinterpret_cast<unsigned long long*>(&sum->total_bits), static_cast<unsigned long long>(wend));
+    }
     // tile_part[t] = last partition with start <= t*1024, for t in [ceil(s/1024), ceil(s_next/1024))
-    if (bad) return;
+    if (!live || bad) return;
     int64_t t0 = (p == 0) ? 0 : (static_cast<int64_t>(s) + kIndexTile - 1) >> kIndexShift;
     int64_t t1 = (p + 1 < P) ? (static_cast<int64_t>(s_next) + kIndexTile - 1) >> kIndexShift : n_index;
     if (t1 > n_index) t1 = n_index;
@@ -755,11 +784,24 @@ __global__ __launch_bounds__(kPackThreads) void k_pack_slow(const T* __restrict_
                                                             int64_t ntiles, uint32_t* __restrict__ words) {
     __shared__ uint32_t s_w[kPackWords];
     __shared__ int64_t s_ww[2];
-    for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        if (tile_path(tinfo[t].info) != TILE_SLOW) continue;
-        const int64_t e0 = t * kIndexTile;
-        const int64_t e1 = (e0 + kIndexTile < n) ? e0 + kIndexTile : n;
-        pack_tile<T>(v, desc, tile_part, e0, e1, words, s_w, s_ww);
+    __shared__ int64_t s_list[kPackThreads];
+    __shared__ int s_cnt;
+    // chunks of kPackThreads tiles: every thread reads one tile's path, the
+    // slow ones are listed in LDS and packed one after another
+    for (int64_t c0 = static_cast<int64_t>(blockIdx.x) * kPackThreads; c0 < ntiles;
+         c0 += static_cast<int64_t>(gridDim.x) * kPackThreads) {
+        if (threadIdx.x == 0) s_cnt = 0;
+        __syncthreads();
+        const int64_t t = c0 + threadIdx.x;
+        if (t < ntiles && tile_path(tinfo[t].info) == TILE_SLOW) s_list[atomicAdd(&s_cnt, 1)] = t;
+        __syncthreads();
+        const int cnt = s_cnt;
+        for (int k = 0; k < cnt; ++k) {
+            const int64_t e0 = s_list[k] * kIndexTile;
+            const int64_t e1 = (e0 + kIndexTile < n) ? e0 + kIndexTile : n;
+            pack_tile<T>(v, desc, tile_part, e0, e1, words, s_w, s_ww);
+        }
+        __syncthreads();  // s_cnt / s_list reused by the next chunk
     }
 }
 
@@ -804,14 +846,29 @@ __global__ __launch_bounds__(kPackThreads) void k_pack_batch(const BatchCol* __r
                                                              int dtype) {
     __shared__ uint32_t s_w[kPackWords];
     __shared__ int64_t s_ww[2];
-    for (int64_t g = blockIdx.x; g < Ttot; g += gridDim.x) {
-        const BatchCol& c = cols[tile_col[g]];
-        if (c.dtype != dtype || c.payload_words > c.words_reserved) continue;
-        const int64_t t = g - c.tile_base;
-        if (tile_path(c.tinfo[t].info) != TILE_SLOW) continue;
-        const int64_t e0 = t * kIndexTile;
-        const int64_t e1 = (e0 + kIndexTile < c.n) ? e0 + kIndexTile : c.n;
-        pack_tile<T>(static_cast<const T*>(c.values), c.desc, c.tile_part, e0, e1, c.words, s_w, s_ww);
+    __shared__ int64_t s_list[kPackThreads];
+    __shared__ int s_cnt;
+    for (int64_t c0 = static_cast<int64_t>(blockIdx.x) * kPackThreads; c0 < Ttot;
+         c0 += static_cast<int64_t>(gridDim.x) * kPackThreads) {
+        if (threadIdx.x == 0) s_cnt = 0;
+        __syncthreads();
+        const int64_t g = c0 + threadIdx.x;
+        if (g < Ttot) {
+            const BatchCol& c = cols[tile_col[g]];
+            if (c.dtype == dtype && c.payload_words <= c.words_reserved &&
+                tile_path(c.tinfo[g - c.tile_base].info) == TILE_SLOW)
+                s_list[atomicAdd(&s_cnt, 1)] = g;
+        }
+        __syncthreads();
+        const int cnt = s_cnt;
+        for (int k = 0; k < cnt; ++k) {
+            const int64_t gk = s_list[k];
+            const BatchCol& c = cols[tile_col[gk]];
+            const int64_t e0 = (gk - c.tile_base) * kIndexTile;
+            const int64_t e1 = (e0 + kIndexTile < c.n) ? e0 + kIndexTile : c.n;
+            pack_tile<T>(static_cast<const T*>(c.values), c.desc, c.tile_part, e0, e1, c.words, s_w, s_ww);
+        }
+        __syncthreads();
     }
 }
 
@@ -830,16 +887,30 @@ __global__ __launch_bounds__(kV6Threads) void k_v6_var(const T* __restrict__ v, 
         const int64_t e = (s + vbs < n) ? s + vbs : n;
         const double dn = static_cast<double>(e - s);
         double sum = 0.0, ssq = 0.0, c1 = 0.0, c2 = 0.0;
-        for (int64_t i = s + tid; i < e; i += kV6Threads) {  // Kahan, PartitioningKernels.cuh:669-682
-            const double val = static_cast<double>(v[i]);
-            const double y1 = val - c1;
-            const double t1 = sum + y1;
-            c1 = (t1 - sum) - y1;
-            sum = t1;
-            const double y2 = __builtin_fma(val, val, -c2);
-            const double t2 = ssq + y2;
-            c2 = (t2 - ssq) - y2;
-            ssq = t2;
+        // Kahan, PartitioningKernels.cuh:669-682, over i = s + tid, s + tid +
+        // 128, ... in that order; eight loads per thread in flight together
+        // (branch-free, the index clamped to e - 1).
+        for (int64_t base = s; base < e; base += 8 * kV6Threads) {
+            T y[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const int64_t i = base + k * kV6Threads + tid;
+                y[k] = v[i < e ? i : e - 1];
+            }
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                if (base + k * kV6Threads + tid < e) {
+                    const double val = static_cast<double>(y[k]);
+                    const double y1 = val - c1;
+                    const double t1 = sum + y1;
+                    c1 = (t1 - sum) - y1;
+                    sum = t1;
+                    const double y2 = __builtin_fma(val, val, -c2);
+                    const double t2 = ssq + y2;
+                    c2 = (t2 - ssq) - y2;
+                    ssq = t2;
+                }
+            }
         }
         for (int o = 16; o > 0; o >>= 1) {  // 32-lane tree of MathHelpers.cuh:91-95
             sum = sum + __shfl_down(sum, o, 32);
@@ -869,43 +940,90 @@ __device__ __forceinline__ float key_float(uint32_t k) {
 }
 
 // Order statistics sorted[nvb/4], sorted[nvb/2], sorted[3nvb/4] by 4-pass
-// radix selection (one workgroup; the reference sorts with thrust::sort).
+// radix selection on 8-bit digits of the order-preserving key (one
+// workgroup; the reference sorts with thrust::sort).  The three ranks are
+// selected together: pass 0 builds one histogram (every prefix is empty),
+// later passes one per rank.  Wave q finds rank q's digit with a 64-lane
+// scan of the 256 bins (4 per lane).  Keys stay in registers when
+// nvb <= 1024 * kSelKeys.
+constexpr int kSelKeys = 16;
+
+__device__ __forceinline__ void sel_count(uint32_t k, int pass, uint32_t hmask, int shift, const uint32_t* pre,
+                                          uint32_t (*hist)[256]) {
+    const uint32_t d = (k >> shift) & 0xFFu;
+    if (pass == 0) {
+        atomicAdd(&hist[0][d], 1u);
+        return;
+    }
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+        if ((k & hmask) == pre[q]) atomicAdd(&hist[q][d], 1u);
+}
+
 __global__ __launch_bounds__(1024) void k_v6_select(const float* __restrict__ var, int64_t nvb,
                                                     float* __restrict__ thr) {
-    __shared__ uint32_t hist[256];
-    __shared__ uint32_t s_prefix;
-    __shared__ int64_t s_rank;
+    __shared__ uint32_t hist[3][256];
+    __shared__ uint32_t s_prefix[3];
+    __shared__ int64_t s_rank[3];
     const int tid = threadIdx.x;
-    for (int q = 0; q < 3; ++q) {
-        const int64_t rank0 = (q == 0) ? nvb / 4 : (q == 1) ? nvb / 2 : 3 * nvb / 4;
-        if (tid == 0) { s_prefix = 0; s_rank = rank0; }
-        __syncthreads();
-        for (int pass = 0; pass < 4; ++pass) {
-            const int shift = 24 - 8 * pass;
-            const uint32_t hmask = (pass == 0) ? 0u : (0xFFFFFFFFu << (32 - 8 * pass));
-            if (tid < 256) hist[tid] = 0;
-            __syncthreads();
-            const uint32_t prefix = s_prefix;
-            for (int64_t i = tid; i < nvb; i += 1024) {
-                const uint32_t k = float_key(var[i]);
-                if ((k & hmask) == prefix) atomicAdd(&hist[(k >> shift) & 0xFF], 1u);
-            }
-            __syncthreads();
-            if (tid == 0) {
-                int64_t r = s_rank;
-                uint32_t dgt = 0;
-                for (; dgt < 256; ++dgt) {
-                    if (r < hist[dgt]) break;
-                    r -= hist[dgt];
-                }
-                s_rank = r;
-                s_prefix = prefix | (dgt << shift);
-            }
-            __syncthreads();
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const bool in_regs = nvb <= 1024LL * kSelKeys;
+    uint32_t keys[kSelKeys];
+    if (in_regs) {
+#pragma unroll
+        for (int k = 0; k < kSelKeys; ++k) {
+            const int64_t i = tid + 1024LL * k;
+            keys[k] = float_key(var[i < nvb ? i : nvb - 1]);
         }
-        if (tid == 0) thr[q] = key_float(s_prefix);
+    }
+    if (tid < 3) {
+        s_prefix[tid] = 0;
+        s_rank[tid] = (tid == 0) ? nvb / 4 : (tid == 1) ? nvb / 2 : 3 * nvb / 4;
+    }
+    for (int pass = 0; pass < 4; ++pass) {
+        const int shift = 24 - 8 * pass;
+        const uint32_t hmask = (pass == 0) ? 0u : (0xFFFFFFFFu << (32 - 8 * pass));
+        if (tid < 3 * 256) (&hist[0][0])[tid] = 0;
+        __syncthreads();
+        const uint32_t pre[3] = {s_prefix[0], s_prefix[1], s_prefix[2]};
+        if (in_regs) {
+#pragma unroll
+            for (int k = 0; k < kSelKeys; ++k)
+                if (tid + 1024LL * k < nvb) sel_count(keys[k], pass, hmask, shift, pre, hist);
+        } else {
+            for (int64_t i = tid; i < nvb; i += 1024) sel_count(float_key(var[i]), pass, hmask, shift, pre, hist);
+        }
+        __syncthreads();
+        if (wave < 3) {
+            const uint32_t* h = hist[pass == 0 ? 0 : wave];
+            const uint32_t h0 = h[4 * lane], h1 = h[4 * lane + 1], h2 = h[4 * lane + 2], h3 = h[4 * lane + 3];
+            const uint32_t sum = h0 + h1 + h2 + h3;
+            uint32_t incl = sum;
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t y = __shfl_up(incl, o, 64);
+                if (lane >= o) incl += y;
+            }
+            const int64_t excl = static_cast<int64_t>(incl) - sum;
+            const int64_t r = s_rank[wave];
+            // the first digit whose running count exceeds r
+            if (r >= excl && r < static_cast<int64_t>(incl)) {
+                int64_t rr = r - excl;
+                uint32_t d = 4 * lane;
+                if (rr >= h0) {
+                    rr -= h0; ++d;
+                    if (rr >= h1) {
+                        rr -= h1; ++d;
+                        if (rr >= h2) { rr -= h2; ++d; }
+                    }
+                }
+                s_rank[wave] = rr;
+                s_prefix[wave] = pre[wave] | (d << shift);
+            }
+        }
         __syncthreads();
     }
+    if (tid < 3) thr[tid] = key_float(s_prefix[tid]);
 }
 
 struct V6Count {
@@ -1018,7 +1136,8 @@ int launch_pack(int dtype, const void* d_values, glc_column* c, hipStream_t s) {
                            static_cast<const T*>(d_values),
                            c->d_desc, c->d_tinfo, c->n, c->d_words);
         GLC_HIP(hipGetLastError());
-        const unsigned gs = static_cast<unsigned>(ntiles < 8192 ? ntiles : 8192);
+        const int64_t chunks = (ntiles + kPackThreads - 1) / kPackThreads;
+        const unsigned gs = static_cast<unsigned>(chunks < 8192 ? (chunks > 0 ? chunks : 1) : 8192);
         hipLaunchKernelGGL(k_pack_slow<T>, dim3(gs), dim3(kPackThreads), 0, s, static_cast<const T*>(d_values),
                            c->d_desc, c->d_tile_part, c->d_tinfo, c->n, ntiles, c->d_words);
         GLC_HIP(hipGetLastError());
@@ -1142,7 +1261,8 @@ int launch_batch_compress(const BatchPlan& b, hipStream_t s) {
     GLC_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_zero_edges_batch, dim3(grid_for(b.Ttot, 256)), dim3(256), 0, s, b.cols, b.tile_col, b.Ttot);
     GLC_HIP(hipGetLastError());
-    const unsigned gp = static_cast<unsigned>(b.Ttot < 8192 ? b.Ttot : 8192);
+    const int64_t pchunks = (b.Ttot + kPackThreads - 1) / kPackThreads;
+    const unsigned gp = static_cast<unsigned>(pchunks < 8192 ? (pchunks > 0 ? pchunks : 1) : 8192);
     for (int dt = GLC_I32; dt <= GLC_U64; ++dt) {
         if (!b.has_dtype[dt]) continue;
         int rc = dispatch_dtype(dt, [&](auto t) {
