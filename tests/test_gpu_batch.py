@@ -145,3 +145,19 @@ def test_batch_errors(gpu):
     empty.compress()
     assert empty.decompress() == []
     assert empty.sizes()[1] == 0
+
+
+def test_batch_mixed_partition_kinds(gpu):
+    """Columns of mixed partition kinds (exact lines, 1-bit, noisy, wide,
+    DIRECT_COPY) in one batch: payloads equal the oracle's column by column."""
+    from test_gpu_parity import mixed_kind_column
+    specs = [(np.int32, 1024), (np.uint64, 1024), (np.int64, 1024), (np.uint64, 1024), (np.int32, 1024)]
+    vals = [mixed_kind_column(dt, 20 * 1024 + 5 * k, 1024, seed=100 + k) for k, (dt, _) in enumerate(specs)]
+    dev = [torch.from_numpy(v).cuda() for v in vals]
+    b = glc.Batch(dev, 1024)
+    b.compress()
+    outs = b.decompress()
+    torch.cuda.synchronize()
+    for k, (v, o) in enumerate(zip(vals, outs)):
+        assert np.array_equal(o.cpu().numpy(), v), k
+        check_view(b.column(k), v, 1024)

@@ -258,3 +258,45 @@ def test_empty_column_and_errors(gpu):
     assert np.array_equal(host(col.decompress_full()), v)
     with pytest.raises(glc.GlcError):
         col.decompress_full(glc.FULL_BITPACKED)
+
+
+def mixed_kind_column(dt, n, S, seed):
+    """Fixed-size partitions of mixed kinds in a random order: exact lines
+    (0-bit partitions, so empty tiles sit between non-empty ones), 1-bit
+    noise, moderate noise, wide residuals (> 32 bits, 64-bit types) and
+    values above 2^53 (DIRECT_COPY, uint64).  Exercises the pack's
+    shared-edge words across every kind of neighbour."""
+    rng = np.random.default_rng(seed)
+    wide = np.dtype(dt).itemsize == 8
+    kinds = ["line", "bit", "noise"] + (["wide"] if wide else []) + (["direct"] if dt == np.uint64 else [])
+    out = np.empty(n, dtype=dt)
+    for s in range(0, n, S):
+        e = min(s + S, n)
+        i = np.arange(e - s, dtype=np.int64)
+        kind = kinds[int(rng.integers(0, len(kinds)))]
+        base = int(rng.integers(1 << 20, 1 << 28))
+        slope = int(rng.integers(0, 1 << 8))
+        if kind == "line":
+            v = base + slope * i
+        elif kind == "bit":
+            v = base + slope * i + rng.integers(0, 2, e - s)
+        elif kind == "noise":
+            v = base + slope * i + rng.integers(-(1 << 12), 1 << 12, e - s)
+        elif kind == "wide":
+            v = (1 << 40) + slope * i + rng.integers(-(1 << 38), 1 << 38, e - s)
+        else:
+            v = None
+        if kind == "direct":
+            out[s:e] = (np.uint64(1) << np.uint64(60)) + rng.integers(0, 1 << 62, e - s).astype(np.uint64)
+        else:
+            out[s:e] = np.asarray(v).astype(dt)
+    return out
+
+
+@pytest.mark.parametrize("dt", [np.int32, np.uint64, np.int64])
+@pytest.mark.parametrize("S", [1024, 512, 2048, 1000, 3000])
+def test_parity_mixed_partition_kinds(gpu, dt, S):
+    n = 40 * 1024 + 37
+    print(f"mixed kinds dtype {np.dtype(dt).name} S {S}", flush=True)
+    v = mixed_kind_column(dt, n, S, seed=S * 7 + np.dtype(dt).itemsize)
+    full_case(v, "fixed", S, fixed_size=S)
