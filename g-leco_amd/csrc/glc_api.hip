@@ -15,6 +15,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <map>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -78,6 +80,26 @@ int32_t fit_block_size(int64_t len0) {
 
 namespace {
 thread_local std::string g_last_error;
+
+// V6 scratch (candidate starts/ends, counts, variances, thresholds), kept per
+// device between calls and grown when a larger column needs more: a
+// per-call hipMalloc/hipFree pair costs hundreds of microseconds and the
+// free synchronises the device.  Held until the process ends; the lock
+// covers the scratch from the first V6 kernel to the last copy out of it.
+struct V6Scratch {
+    std::mutex mu;
+    void* p = nullptr;
+    int64_t bytes = 0;
+};
+
+V6Scratch& v6_scratch(int device) {
+    static std::mutex table_mu;
+    static std::map<int, V6Scratch*>* table = new std::map<int, V6Scratch*>();
+    std::lock_guard<std::mutex> g(table_mu);
+    V6Scratch*& e = (*table)[device];
+    if (!e) e = new V6Scratch();
+    return *e;
+}
 
 int alloc_column(glc_column* c, int64_t P, hipStream_t s) {
     c->P = P;
@@ -193,12 +215,24 @@ int compress_impl(int dtype, const void* d_values, int64_t n, int partitioner, i
             // temp: start[cap], end[cap], count[nvb+1] (int64), var[nvb], thr[3]
             const int64_t bytes = align_up(cap * 4, 256) * 2 + align_up((nvb + 1) * 8, 256) +
                                   align_up(nvb * 4, 256) + 256;
-            void* tmp = nullptr;
-            if (hipMalloc(&tmp, bytes) != hipSuccess) {
-                (void)hipGetLastError();
-                return bail(fail(GLC_ERR_OOM, "V6 scratch allocation failed"));
+            V6Scratch& sc = v6_scratch(c->device);
+            std::lock_guard<std::mutex> scratch_lock(sc.mu);
+            if (sc.bytes < bytes) {
+                if (sc.p) {
+                    GLC_HIP(hipStreamSynchronize(s));
+                    (void)hipFree(sc.p);
+                }
+                sc.p = nullptr;
+                sc.bytes = 0;
+                const int64_t want = align_up(bytes + bytes / 4, 1 << 20);
+                if (hipMalloc(&sc.p, want) != hipSuccess) {
+                    (void)hipGetLastError();
+                    sc.p = nullptr;
+                    return bail(fail(GLC_ERR_OOM, "V6 scratch allocation failed"));
+                }
+                sc.bytes = want;
             }
-            char* t = static_cast<char*>(tmp);
+            char* t = static_cast<char*>(sc.p);
             int32_t* ts = reinterpret_cast<int32_t*>(t);
             int32_t* te = reinterpret_cast<int32_t*>(t + align_up(cap * 4, 256));
             int64_t* tc = reinterpret_cast<int64_t*>(t + 2 * align_up(cap * 4, 256));
@@ -216,7 +250,6 @@ int compress_impl(int dtype, const void* d_values, int64_t n, int partitioner, i
             if (!rc && hipMemcpyAsync(c->d_start, ts, P * 4, hipMemcpyDeviceToDevice, s) != hipSuccess) rc = GLC_ERR_HIP;
             if (!rc && hipMemcpyAsync(c->d_end, te, P * 4, hipMemcpyDeviceToDevice, s) != hipSuccess) rc = GLC_ERR_HIP;
             if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = GLC_ERR_HIP;
-            (void)hipFree(tmp);
             if (rc) return bail(rc == GLC_ERR_HIP ? fail(GLC_ERR_HIP, "V6 partitioning failed") : rc);
             len0 = static_cast<int64_t>(first[1]) - first[0];
             break;
