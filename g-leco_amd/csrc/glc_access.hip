@@ -98,14 +98,95 @@ __device__ __forceinline__ T ra_lookup(const Desc* __restrict__ desc, const int3
     }
 }
 
+// kRaBatch lookups per thread with their loads staged: the positions, then
+// the tiles' 32-byte records, then the payload words (or pre-unpacked
+// deltas) of all of them are in flight together, so a thread pays three
+// dependent memory round trips per batch instead of per lookup.  Lookup
+// j0 + k * kThreads of the thread's batch is its k-th (coalesced positions
+// and results).  Out-of-range positions read record 0 / word 0 and return 0;
+// tiles of the partition-spanning path take ra_lookup.
+constexpr int kRaBatch = 4;
+
 template <class T, bool POS64, bool PLAIN>
 __global__ __launch_bounds__(kThreads) void k_random_access(RaArgs a) {
     T* __restrict__ out = static_cast<T*>(a.out);
-    const int64_t stride = static_cast<int64_t>(gridDim.x) * kThreads;
-    for (int64_t j = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x; j < a.q; j += stride) {
-        const int64_t e = POS64 ? static_cast<const int64_t*>(a.pos)[j]
-                                : static_cast<int64_t>(static_cast<const int32_t*>(a.pos)[j]);
-        out[j] = ra_lookup<T, PLAIN>(a.desc, a.tile_part, a.tinfo, a.words, a.plain, a.params, a.n, a.fixed, e);
+    const int64_t per_block = static_cast<int64_t>(kThreads) * kRaBatch;
+    const int64_t step = static_cast<int64_t>(gridDim.x) * per_block;
+    for (int64_t j0 = static_cast<int64_t>(blockIdx.x) * per_block + threadIdx.x; j0 < a.q; j0 += step) {
+        int64_t e[kRaBatch];
+#pragma unroll
+        for (int k = 0; k < kRaBatch; ++k) {
+            const int64_t j = j0 + k * kThreads;
+            e[k] = (j < a.q) ? (POS64 ? static_cast<const int64_t*>(a.pos)[j]
+                                      : static_cast<int64_t>(static_cast<const int32_t*>(a.pos)[j]))
+                             : -1;
+        }
+        // fixed-size lookups, empty columns and a released payload (all
+        // widths 0): one at a time through ra_lookup
+        if (a.fixed > 0 || a.n <= 0 || (!PLAIN && a.words == nullptr)) {
+#pragma unroll
+            for (int k = 0; k < kRaBatch; ++k) {
+                const int64_t j = j0 + k * kThreads;
+                if (j < a.q)
+                    out[j] = ra_lookup<T, PLAIN>(a.desc, a.tile_part, a.tinfo, a.words, a.plain, a.params, a.n,
+                                                 a.fixed, e[k]);
+            }
+            continue;
+        }
+        bool in[kRaBatch];
+        TileInfo x[kRaBatch];
+#pragma unroll
+        for (int k = 0; k < kRaBatch; ++k) {
+            in[k] = e[k] >= 0 && e[k] < a.n;
+            x[k] = a.tinfo[in[k] ? (e[k] >> kIndexShift) : 0];
+        }
+        uint32_t w0[kRaBatch], w1[kRaBatch], w2[kRaBatch];
+        int64_t pl[kRaBatch];
+#pragma unroll
+        for (int k = 0; k < kRaBatch; ++k) {
+            const bool fast = in[k] && tile_path(x[k].info) != TILE_SLOW;
+            if constexpr (PLAIN) {
+                pl[k] = a.plain[fast ? e[k] : 0];
+            } else {
+                const int b = tile_bits(x[k].info);
+                const int64_t bp = fast ? x[k].W + (e[k] & (kIndexTile - 1)) * b : 0;
+                const int64_t wi = bp >> 5;  // + 2 stays inside the payload's zero pad
+                w0[k] = a.words[wi];
+                w1[k] = a.words[wi + 1];
+                w2[k] = (b > 32) ? a.words[wi + 2] : 0u;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kRaBatch; ++k) {
+            const int64_t j = j0 + k * kThreads;
+            if (j >= a.q) continue;
+            T v = T(0);
+            if (in[k]) {
+                const TileInfo& xk = x[k];
+                if (tile_path(xk.info) == TILE_SLOW) {
+                    v = ra_lookup<T, PLAIN>(a.desc, a.tile_part, a.tinfo, a.words, a.plain, a.params, a.n, a.fixed,
+                                            e[k]);
+                } else {
+                    const int b = tile_bits(xk.info);
+                    const int32_t type = tile_type(xk.info);
+                    const int32_t i = xk.lbase + static_cast<int32_t>(e[k] & (kIndexTile - 1));
+                    if constexpr (PLAIN) {
+                        v = (type == MODEL_DIRECT_COPY) ? static_cast<T>(pl[k])
+                                                        : apply_delta<T>(dec_pred<T>(xk.t0, xk.t1, i), pl[k]);
+                    } else {
+                        // extract_raw_g on the words already loaded
+                        const uint32_t sh = static_cast<uint32_t>((xk.W + (e[k] & (kIndexTile - 1)) * b) & 31);
+                        uint64_t raw = ((static_cast<uint64_t>(w1[k]) << 32) | w0[k]) >> sh;
+                        if (sh + b > 64) raw |= static_cast<uint64_t>(w2[k]) << (64 - sh);
+                        if (b < 64) raw &= (1ULL << b) - 1ULL;
+                        if (b == 0) raw = 0;
+                        v = (type == MODEL_DIRECT_COPY) ? static_cast<T>(raw)
+                                                        : apply_delta<T>(dec_pred<T>(xk.t0, xk.t1, i), sign_extend(raw, b));
+                    }
+                }
+            }
+            out[j] = v;
+        }
     }
 }
 
@@ -184,8 +265,8 @@ template <class F> int dispatch_dtype(int dtype, F&& f) {
     }
 }
 
-unsigned ra_grid(int64_t q) {
-    int64_t g = (q + kThreads - 1) / kThreads;
+unsigned ra_grid(int64_t q, int per_thread = 1) {
+    int64_t g = (q + static_cast<int64_t>(kThreads) * per_thread - 1) / (static_cast<int64_t>(kThreads) * per_thread);
     const int64_t cap = 256LL * 64;  // plenty of waves to cover HBM latency; grid-stride beyond
     if (g > cap) g = cap;
     if (g < 1) g = 1;
@@ -229,7 +310,7 @@ int launch_random_access(const glc_column* c, const void* d_pos, int pos64, int6
     if (c->P == 0) a.n = 0;  // every query decodes to 0
     return dispatch_dtype(c->dtype, [&](auto t) {
         using T = decltype(t);
-        const dim3 g(ra_grid(q));
+        const dim3 g(ra_grid(q, kRaBatch));
         if (pos64) {
             if (plain) hipLaunchKernelGGL((k_random_access<T, true, true>), g, dim3(kThreads), 0, s, a);
             else hipLaunchKernelGGL((k_random_access<T, true, false>), g, dim3(kThreads), 0, s, a);
