@@ -31,6 +31,8 @@
 #include <map>
 #include <memory>
 #include <mutex>
+
+#include <sys/prctl.h>
 #include <string>
 #include <thread>
 #include <unordered_map>
@@ -365,6 +367,10 @@ int admit_and_launch(glc_server* s) {
 }
 
 void worker_main(glc_server* s) {
+    // The worker polls in-flight steps with short sleeps; the default 50 us
+    // timer slack would stretch each 5 us sleep to ~55 us and delay every
+    // completion by that much.  1 us slack, for this thread only.
+    (void)prctl(PR_SET_TIMERSLACK, 1000UL, 0UL, 0UL, 0UL);
     if (hipSetDevice(s->device) != hipSuccess) {
         std::lock_guard<std::mutex> lk(s->mu);
         s->worker_rc = GLC_ERR_HIP;
