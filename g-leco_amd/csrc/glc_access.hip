@@ -192,7 +192,14 @@ __global__ __launch_bounds__(kThreads) void k_random_access(RaArgs a) {
 
 // Lookups of many requests over many columns in one launch: lookup j
 // belongs to the request r with reqs[r].first <= j < reqs[r+1].first.
-__global__ __launch_bounds__(kThreads) void k_serve_lookup(const ServeReq* __restrict__ reqs, int32_t R, int64_t Q) {
+// The step's table and positions sit in pinned host memory and are read
+// over the bus: each workgroup copies the table (R <= kServeMaxReqs rows)
+// into LDS once, and every position is read once, coalesced.
+__global__ __launch_bounds__(kThreads) void k_serve_lookup(const ServeReq* __restrict__ reqs_host, int32_t R,
+                                                           int64_t Q) {
+    __shared__ ServeReq reqs[kServeMaxReqs];
+    for (int i = threadIdx.x; i < R; i += kThreads) reqs[i] = reqs_host[i];
+    __syncthreads();
     const int64_t stride = static_cast<int64_t>(gridDim.x) * kThreads;
     for (int64_t j = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x; j < Q; j += stride) {
         int32_t lo = 0, hi = R - 1;
@@ -275,9 +282,10 @@ unsigned ra_grid(int64_t q, int per_thread = 1) {
 
 }  // namespace
 
-int launch_serve_lookup(const ServeReq* d_reqs, int32_t R, int64_t Q, hipStream_t s) {
+int launch_serve_lookup(const ServeReq* reqs, int32_t R, int64_t Q, hipStream_t s) {
+    if (R < 1 || R > kServeMaxReqs) return fail(GLC_ERR_INVALID_ARG, "serving step: request count out of range");
     if (Q <= 0 || R <= 0) return GLC_OK;
-    hipLaunchKernelGGL(k_serve_lookup, dim3(ra_grid(Q)), dim3(kThreads), 0, s, d_reqs, R, Q);
+    hipLaunchKernelGGL(k_serve_lookup, dim3(ra_grid(Q)), dim3(kThreads), 0, s, reqs, R, Q);
     GLC_HIP(hipGetLastError());
     return GLC_OK;
 }

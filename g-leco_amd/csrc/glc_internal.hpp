@@ -146,14 +146,15 @@ struct ServeReq {
     const TileInfo* tinfo;
     const uint32_t* words;
     const double* params;
-    const int32_t* pos;  // the request's positions (device)
+    const int32_t* pos;  // the request's positions (pinned host memory, device-visible)
     void* out;           // the request's results (device)
     int64_t n;
     int64_t first;       // index of the request's first lookup in the step
     int32_t dtype;
     int32_t fixed;       // fixed partition size, 0 = generic lookup
 };
-int launch_serve_lookup(const ServeReq* d_reqs, int32_t R, int64_t Q, hipStream_t s);
+constexpr int kServeMaxReqs = 256;  // lookup requests per serving step (the kernel's LDS table)
+int launch_serve_lookup(const ServeReq* reqs, int32_t R, int64_t Q, hipStream_t s);
 
 int launch_batch_compress(const BatchPlan& b, hipStream_t s);
 int launch_batch_decode(const BatchPlan& b, int wpg, int tpw, hipStream_t s);

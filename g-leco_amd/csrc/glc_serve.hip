@@ -123,8 +123,8 @@ struct Req {
 // A lookup staging slot: pinned + device buffers for one step's request
 // table and positions, its stream, and the event of its step.
 struct Slot {
-    char* h = nullptr;
-    char* d = nullptr;
+    char* h = nullptr;  // pinned host staging: the step's table, then its positions
+    char* d = nullptr;  // the same bytes as the device sees them
     hipStream_t stream = nullptr;
     hipEvent_t ev = nullptr;
     bool busy = false;
@@ -269,7 +269,12 @@ int launch_step(glc_server* s, Slot& sl, std::vector<Req*>& look) {
         std::memcpy(hpos + first, r->pos.data(), 4 * r->count);
         first += r->count;
     }
-    GLC_HIP(hipMemcpyAsync(sl.d, sl.h, tbl + 4 * Q, hipMemcpyHostToDevice, sl.stream));
+    // The kernel reads the table and positions from the pinned slot itself:
+    // an H2D copy of a multi-MB step was seen to block this thread for
+    // milliseconds inside hipMemcpyAsync, stalling every request behind it.
+    // The fence makes the stores above (memcpy may use non-temporal ones)
+    // globally visible before the launch.
+    std::atomic_thread_fence(std::memory_order_seq_cst);
     int rc = glc::launch_serve_lookup(reinterpret_cast<const glc::ServeReq*>(sl.d), R, Q, sl.stream);
     if (rc) return rc;
     GLC_HIP(hipEventRecord(sl.ev, sl.stream));
@@ -315,7 +320,8 @@ int admit_and_launch(glc_server* s) {
         for (auto it = s->pending.begin(); it != s->pending.end();) {
             Req* r = *it;
             if (s->inflight >= s->max_inflight) break;
-            if (r->kind == RK_LOOKUP && (!slot || slot_need(R + 1, Q + r->count) > s->slot_bytes)) {
+            if (r->kind == RK_LOOKUP &&
+                (!slot || R + 1 > glc::kServeMaxReqs || slot_need(R + 1, Q + r->count) > s->slot_bytes)) {
                 ++it;  // next step
                 continue;
             }
@@ -487,8 +493,7 @@ int glc_server_create(int64_t hbm_budget_bytes, int64_t staging_bytes, int32_t m
     if (staging_bytes <= 0) staging_bytes = 3 * ((16LL << 20) + (1 << 20));
     const int64_t slot = glc::align_up(staging_bytes / kLookupSlots, kAlign);
     if (slot < slot_need(1, 1)) return fail(GLC_ERR_INVALID_ARG, "staging too small");
-    if (hbm_budget_bytes < kLookupSlots * slot + (1 << 20))
-        return fail(GLC_ERR_INVALID_ARG, "HBM budget must exceed the staging by at least 1 MiB");
+    if (hbm_budget_bytes < (1 << 20)) return fail(GLC_ERR_INVALID_ARG, "HBM budget must be at least 1 MiB");
     auto* s = new glc_server();
     s->max_inflight = max_inflight;
     s->slot_bytes = slot;
@@ -508,19 +513,24 @@ int glc_server_create(int64_t hbm_budget_bytes, int64_t staging_bytes, int32_t m
         shutdown(s);
         return fail(GLC_ERR_OOM, "pinned staging could not be allocated");
     }
-    char* d = static_cast<char*>(s->dmem);
+    void* pinned_dev = nullptr;
+    if (hipHostGetDevicePointer(&pinned_dev, s->pinned, 0) != hipSuccess || !pinned_dev) {
+        (void)hipGetLastError();
+        shutdown(s);
+        return fail(GLC_ERR_HIP, "pinned staging is not visible to the device");
+    }
     for (int i = 0; i < kLookupSlots; ++i) {
         Slot& sl = s->slots[i];
         sl.h = s->pinned + i * slot;
-        sl.d = d + i * slot;
+        sl.d = static_cast<char*>(pinned_dev) + i * slot;
         if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||
             hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming) != hipSuccess) {
             shutdown(s);
             return fail(GLC_ERR_HIP, "stream/event creation failed");
         }
     }
-    s->pool = d + kLookupSlots * slot;
-    s->dev.reset((hbm_budget_bytes - kLookupSlots * slot) / kAlign * kAlign);
+    s->pool = static_cast<char*>(s->dmem);
+    s->dev.reset(hbm_budget_bytes / kAlign * kAlign);
     if (hipStreamCreateWithFlags(&s->s_dec, hipStreamNonBlocking) != hipSuccess) {
         shutdown(s);
         return fail(GLC_ERR_HIP, "stream creation failed");

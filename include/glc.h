@@ -259,14 +259,15 @@ typedef struct {
     double p50_lookup_ms, p99_lookup_ms;
     double p50_decompress_ms, p99_decompress_ms;
     double elapsed_s;             /* first submit -> last completion */
-    int64_t pool_bytes;           /* result pool (budget minus staging) */
+    int64_t pool_bytes;           /* result pool (the HBM budget) */
     int64_t pool_peak_bytes;
     int32_t inflight_peak;
     int32_t worker_status;        /* GLC_OK, or the error that stopped the worker */
 } glc_server_stats;
-/* hbm_budget_bytes: device memory the server may use (lookup staging +
- * result pool); staging_bytes: pinned/device staging for lookup positions
- * (0 = 51 MiB, three slots); max_inflight: admitted-but-unreleased cap. */
+/* hbm_budget_bytes: device memory of the result pool (>= 1 MiB);
+ * staging_bytes: pinned host staging for each step's request table and
+ * positions, read by the lookup kernel over the bus (0 = 51 MiB, three
+ * slots); max_inflight: admitted-but-unreleased cap. */
 int glc_server_create(int64_t hbm_budget_bytes, int64_t staging_bytes, int32_t max_inflight, glc_server** out);
 /* Register a resident column (not owned; must outlive the server). */
 int glc_server_add_column(glc_server* s, const glc_column* col, int32_t* col_id);

@@ -108,7 +108,7 @@ def test_batch_and_server_argument_checks_need_no_device():
     assert L.glc_batch_create(-1, None, None, None, 1024, C.byref(h)) == -1
     assert L.glc_batch_create(1, None, None, None, 1024, C.byref(h)) == -1
     assert L.glc_server_create(1 << 30, 0, 0, C.byref(h)) == -1      # max_inflight < 1
-    assert L.glc_server_create(1 << 20, 0, 16, C.byref(h)) == -1     # budget below the staging
+    assert L.glc_server_create((1 << 20) - 1, 0, 16, C.byref(h)) == -1  # budget below 1 MiB
     assert L.glc_server_create(1 << 30, 16, 16, C.byref(h)) == -1    # staging too small
     assert not h.value
 

@@ -109,7 +109,7 @@ def test_admission_by_budget(gpu):
     col = glc.compress(torch.from_numpy(v).cuda(), glc.PART_V6, 0, 2048)
     torch.cuda.synchronize()
     staging = 3 << 20
-    srv = glc.Server(staging + (9 << 20), max_inflight=64, staging_bytes=staging)  # room for two results
+    srv = glc.Server(9 << 20, max_inflight=64, staging_bytes=staging)  # a 9 MiB pool: room for two results
     cid = srv.add_column(col)
     big = W.c2_uint64(1 << 21, seed=22)
     col_big = glc.compress(torch.from_numpy(big).cuda(), glc.PART_V6, 0, 2048)
