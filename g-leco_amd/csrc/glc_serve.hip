@@ -111,7 +111,8 @@ struct Req {
     int32_t col = 0;
     const glc_column* c = nullptr;  // the resident column (fixed at submit)
     int64_t count = 0;            // lookups or values
-    std::vector<int32_t> pos;     // lookup positions (host copy)
+    std::vector<int32_t> pos;     // lookup positions (host copy) when the arena had no room
+    int64_t pos_off = -1;         // else: offset of the positions in the pinned position arena
     int64_t out_off = -1;         // result block in the pool
     int64_t out_bytes = 0;
     int bypassed = 0;
@@ -143,11 +144,17 @@ struct Col {
 struct glc_server {
     int device = 0;
     int32_t max_inflight = 0;
-    void* dmem = nullptr;  // device memory of the budget: slot staging, then the result pool
+    void* dmem = nullptr;  // device memory of the budget: the result pool
     char* pool = nullptr;
     RangeAlloc dev;        // result blocks inside the pool
     char* pinned = nullptr;
     int64_t slot_bytes = 0;
+    // Pinned position arena: a lookup's positions are copied here by the
+    // submitting thread and read by the kernel in place, so the worker copies
+    // nothing for them (requests that find no room go through the slots).
+    char* parena = nullptr;
+    char* parena_dev = nullptr;
+    RangeAlloc arena;
     Slot slots[kLookupSlots];
     hipStream_t s_dec = nullptr;
     std::vector<Col> cols;
@@ -191,6 +198,10 @@ int get_event(glc_server* s, hipEvent_t* ev) {
 }
 
 void complete(glc_server* s, Req* r, int rc, double t) {
+    if (r->pos_off >= 0) {  // the kernel that read the positions has finished
+        s->arena.release(r->pos_off, 4 * r->count);
+        r->pos_off = -1;
+    }
     r->t_done = t;
     r->rc = rc;
     r->state = rc == GLC_OK ? RQ_DONE : RQ_FAILED;
@@ -249,7 +260,7 @@ int launch_step(glc_server* s, Slot& sl, std::vector<Req*>& look) {
     auto* tab = reinterpret_cast<glc::ServeReq*>(sl.h);
     int32_t* hpos = reinterpret_cast<int32_t*>(sl.h + tbl);
     const int32_t* dpos = reinterpret_cast<const int32_t*>(sl.d + tbl);
-    int64_t first = 0;
+    int64_t first = 0, in_slot = 0;
     for (int32_t i = 0; i < R; ++i) {
         Req* r = look[i];
         const glc_column* c = r->c;
@@ -259,14 +270,17 @@ int launch_step(glc_server* s, Slot& sl, std::vector<Req*>& look) {
         q.tinfo = c->d_tinfo;
         q.words = c->d_words;
         q.params = c->d_params;
-        q.pos = dpos + first;
+        q.pos = (r->pos_off >= 0) ? reinterpret_cast<const int32_t*>(s->parena_dev + r->pos_off) : dpos + in_slot;
         q.out = s->pool + r->out_off;
         q.n = c->n;
         q.first = first;
         q.dtype = c->dtype;
         q.fixed = 0;
         tab[i] = q;
-        std::memcpy(hpos + first, r->pos.data(), 4 * r->count);
+        if (r->pos_off < 0) {
+            std::memcpy(hpos + in_slot, r->pos.data(), 4 * r->count);
+            in_slot += r->count;
+        }
         first += r->count;
     }
     // The kernel reads the table and positions from the pinned slot itself:
@@ -316,12 +330,13 @@ int admit_and_launch(glc_server* s) {
                 slot = &sl;
                 break;
             }
-        int64_t R = 0, Q = 0;
+        int64_t R = 0, Q = 0;  // Q: positions that go through the slot
         for (auto it = s->pending.begin(); it != s->pending.end();) {
             Req* r = *it;
             if (s->inflight >= s->max_inflight) break;
+            const int64_t q_slot = (r->kind == RK_LOOKUP && r->pos_off < 0) ? r->count : 0;
             if (r->kind == RK_LOOKUP &&
-                (!slot || R + 1 > glc::kServeMaxReqs || slot_need(R + 1, Q + r->count) > s->slot_bytes)) {
+                (!slot || R + 1 > glc::kServeMaxReqs || slot_need(R + 1, Q + q_slot) > s->slot_bytes)) {
                 ++it;  // next step
                 continue;
             }
@@ -339,7 +354,7 @@ int admit_and_launch(glc_server* s) {
             s->pool_peak = std::max(s->pool_peak, s->dev.used());
             if (r->kind == RK_LOOKUP) {
                 R += 1;
-                Q += r->count;
+                Q += q_slot;
                 look.push_back(r);
             } else {
                 dec.push_back(r);
@@ -434,16 +449,16 @@ float pct(std::vector<float> v, double f) {
 }
 
 int submit(glc_server* s, Req* r, int64_t* out_id) {
-    if (glc::align_up(r->out_bytes, kAlign) > s->dev.size() ||
-        (r->kind == RK_LOOKUP && slot_need(1, r->count) > s->slot_bytes)) {
-        delete r;
-        return fail(GLC_ERR_BUDGET, "request needs more memory than the serving budget / staging allows");
-    }
     std::lock_guard<std::mutex> lk(s->mu);
-    if (s->stop) {
+    auto drop = [&](int code, const char* msg) {
+        if (r->pos_off >= 0) s->arena.release(r->pos_off, 4 * r->count);
         delete r;
-        return fail(GLC_ERR_STATE, "server is shutting down");
-    }
+        return fail(code, msg);
+    };
+    if (glc::align_up(r->out_bytes, kAlign) > s->dev.size() ||
+        (r->kind == RK_LOOKUP && slot_need(1, r->count) > s->slot_bytes))
+        return drop(GLC_ERR_BUDGET, "request needs more memory than the serving budget / staging allows");
+    if (s->stop) return drop(GLC_ERR_STATE, "server is shutting down");
     r->id = s->next_id++;
     r->t_submit = now_s();
     s->reqs.emplace(r->id, std::unique_ptr<Req>(r));
@@ -479,6 +494,7 @@ void shutdown(glc_server* s) {
     if (s->s_dec) (void)hipStreamDestroy(s->s_dec);
     if (s->dmem) (void)hipFree(s->dmem);
     if (s->pinned) (void)hipHostFree(s->pinned);
+    if (s->parena) (void)hipHostFree(s->parena);
     delete s;
 }
 
@@ -513,6 +529,16 @@ int glc_server_create(int64_t hbm_budget_bytes, int64_t staging_bytes, int32_t m
         shutdown(s);
         return fail(GLC_ERR_OOM, "pinned staging could not be allocated");
     }
+    const int64_t arena_bytes = kLookupSlots * slot;
+    void* arena_dev = nullptr;
+    if (hipHostMalloc(reinterpret_cast<void**>(&s->parena), static_cast<size_t>(arena_bytes)) != hipSuccess ||
+        hipHostGetDevicePointer(&arena_dev, s->parena, 0) != hipSuccess || !arena_dev) {
+        (void)hipGetLastError();
+        shutdown(s);
+        return fail(GLC_ERR_OOM, "pinned position arena could not be set up");
+    }
+    s->parena_dev = static_cast<char*>(arena_dev);
+    s->arena.reset(arena_bytes);
     void* pinned_dev = nullptr;
     if (hipHostGetDevicePointer(&pinned_dev, s->pinned, 0) != hipSuccess || !pinned_dev) {
         (void)hipGetLastError();
@@ -565,8 +591,19 @@ int glc_server_submit_lookup(glc_server* s, int32_t col_id, const int32_t* h_pos
     r->col = col_id;
     r->c = c;
     r->count = q;
-    r->pos.assign(h_pos, h_pos + q);
     r->out_bytes = q * c->elem_size;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (slot_need(1, q) <= s->slot_bytes) r->pos_off = s->arena.alloc(4 * q);
+    }
+    if (r->pos_off >= 0) {
+        std::memcpy(s->parena + r->pos_off, h_pos, 4 * q);
+        // the worker launches the kernel that reads these bytes: make this
+        // thread's stores (possibly non-temporal) globally visible first
+        std::atomic_thread_fence(std::memory_order_seq_cst);
+    } else {
+        r->pos.assign(h_pos, h_pos + q);
+    }
     return submit(s, r, req_id);
 }
 
