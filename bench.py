@@ -407,21 +407,26 @@ def bench_c5(args):
         srv.reset_stats()
         t0 = time.perf_counter()
 
-        def client():
-            for k, (kind, c, pos) in enumerate(reqs):
+        nth = max(1, args.c5_clients)
+
+        def client(t):  # submitting thread t takes requests t, t + nth, ... at their arrival times
+            for k in range(t, len(reqs), nth):
+                kind, c, pos = reqs[k]
                 ta = t0 + arr[k]
                 while time.perf_counter() < ta:
                     time.sleep(0)
                 rid[k] = srv.submit_decompress(ids[c]) if kind == "dec" else srv.submit_lookup(ids[c], pos)
 
-        th = threading.Thread(target=client)
-        th.start()
+        ths = [threading.Thread(target=client, args=(t,)) for t in range(nth)]
+        for th in ths:
+            th.start()
         for k in range(len(reqs)):
             while rid[k] is None:
                 time.sleep(20e-6)
             srv.wait(rid[k], 120000)
             srv.release(rid[k])
-        th.join()
+        for th in ths:
+            th.join()
         wall = time.perf_counter() - t0
         st = srv.stats()
         return {"p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "p99_lookup_ms": st["p99_lookup_ms"],
@@ -461,7 +466,8 @@ def bench_c5(args):
         "vs_baseline": None, "dtype": "u64/i64", "data": "synthetic (seeded, generated on the GPU)",
         "config": {"workload": f"c5_serving_{C}_columns_2^{lo}..2^{hi}", "columns": C, "values": int(lens.sum()),
                    "compressed_gib": round(comp / 2**30, 3), "requests": len(reqs), "lookups": int(nlook),
-                   "budget_gib": args.c5_budget_gib, "max_inflight": args.c5_max_inflight},
+                   "budget_gib": args.c5_budget_gib, "max_inflight": args.c5_max_inflight,
+                   "served_client_threads": args.c5_clients},
         "plain_capacity_req_per_s": round(mu, 1),
         "speedup_vs_plain_at_equal_p99": round(best["req_per_s"] / rows["plain_0.8mu"]["req_per_s"], 3) if best else None,
         "rows": {k: {kk: (round(vv, 3) if isinstance(vv, float) else vv) for kk, vv in v.items()} for k, v in rows.items()},
@@ -609,6 +615,7 @@ def main():
     ap.add_argument("--c5-probe", type=int, default=200)
     ap.add_argument("--c5-budget-gib", type=float, default=16.0)
     ap.add_argument("--c5-max-inflight", type=int, default=256)
+    ap.add_argument("--c5-clients", type=int, default=2, help="threads submitting the served rows' requests")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain device copy of the input (known bytes) for counter calibration")
     ap.add_argument("--c4-columns", type=int, default=10_000)
