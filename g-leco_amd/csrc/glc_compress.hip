@@ -529,18 +529,30 @@ __global__ void k_build_tinfo(const Desc* __restrict__ desc, const int32_t* __re
 __global__ void k_build_index_batch(const BatchCol* __restrict__ cols, const int32_t* __restrict__ part_col,
                                     int64_t Ptot) {
     const int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (g >= Ptot) return;
-    const BatchCol& c = cols[part_col[g]];
+    const bool live = g < Ptot;  // every lane stays for the wave vote below
+    const int32_t ci = live ? part_col[g] : -1;
+    int32_t b = INT32_MIN;
+    if (live) b = cols[ci].bits[g - cols[ci].part_base];
+    // a wave whose partitions all belong to one column (the usual case:
+    // partitions are consecutive) issues one atomic for that column
+    const int32_t c0 = __builtin_amdgcn_readfirstlane(live ? ci : INT32_MAX);
+    if (__all(!live || ci == c0)) {
+        const int32_t wb = wave_max_all(b);
+        if ((threadIdx.x & 63) == 0 && c0 != INT32_MAX) atomicMax(&cols[c0].sum->max_bits, wb);
+    } else if (live) {
+        atomicMax(&cols[ci].sum->max_bits, b);
+    }
+    if (!live) return;
+    const BatchCol& c = cols[ci];
     const int64_t p = g - c.part_base;
     const int32_t s = c.start[p], e = c.end[p];
     Desc d;
     d.theta0 = c.params[4 * p];
     d.theta1 = c.params[4 * p + 1];
-    d.packed = desc_pack(c.off[p], c.bits[p], c.type[p]);
+    d.packed = desc_pack(c.off[p], b, c.type[p]);
     d.start = s;
     d.len = e > s ? e - s : 0;
     c.desc[p] = d;
-    atomicMax(&c.sum->max_bits, c.bits[p]);
     const int32_t s_next = (p + 1 < c.P) ? c.start[p + 1] : 0x7FFFFFFF;
     const int64_t t0 = (p == 0) ? 0 : (static_cast<int64_t>(s) + kIndexTile - 1) >> kIndexShift;
     int64_t t1 = (p + 1 < c.P) ? (static_cast<int64_t>(s_next) + kIndexTile - 1) >> kIndexShift : c.n_index;
