@@ -128,3 +128,27 @@ def test_admission_by_budget(gpu):
     assert st["pool_peak_bytes"] <= st["pool_bytes"]
     assert st["inflight_peak"] <= 2
     srv.close()
+
+
+def test_lookups_when_the_position_arena_is_full(gpu):
+    """Positions go into the pinned position arena at submit; requests that
+    find it full keep a host copy and go through a staging slot.  With a 3 MiB
+    staging (a 3 MiB arena, 1 MiB slots) and 24 concurrent 100k-position
+    lookups (400 KB each) both paths are taken; every result must equal the
+    expected values."""
+    vals, cols = columns()
+    srv = glc.Server(256 << 20, max_inflight=512, staging_bytes=3 << 20)
+    ids = [srv.add_column(c) for c in cols]
+    rng = np.random.default_rng(5)
+    reqs = []
+    for k in range(24):
+        c = k % len(cols)
+        reqs.append((c, rng.integers(0, vals[c].size, 100_000).astype(np.int32)))
+    rids = [srv.submit_lookup(ids[c], pos) for c, pos in reqs]  # no waiting between submissions
+    for rid, (c, pos) in zip(rids, reqs):
+        srv.wait(rid, 60000)
+        assert np.array_equal(srv.result(rid, ids[c]), vals[c][pos])
+        srv.release(rid)
+    st = srv.stats()
+    assert st["completed"] == len(reqs) and st["worker_status"] == 0
+    srv.close()
