@@ -288,21 +288,25 @@ __global__ __launch_bounds__(kFitThreads) void k_fit_batch(const BatchCol* __res
     }
 }
 
-// Batched fit, one wave per partition of the columns whose block size is 256.
-template <class T>
+// Batched fit, one wave per partition of the columns whose block size is
+// 256, every element type in one launch (the type comes from the column).
 __global__ __launch_bounds__(64 * kFitWaves) void k_fit_wave_batch(const BatchCol* __restrict__ cols,
-                                                                  const int32_t* __restrict__ part_col, int64_t Ptot,
-                                                                  int dtype) {
+                                                                  const int32_t* __restrict__ part_col, int64_t Ptot) {
     const int lane = threadIdx.x & 63;
     const int64_t g = static_cast<int64_t>(blockIdx.x) * kFitWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (g >= Ptot) return;
     const BatchCol& c = cols[part_col[g]];
-    if (c.dtype != dtype || c.fit_bs != kFitWaveBs) return;
+    if (c.fit_bs != kFitWaveBs) return;
     const int64_t p = g - c.part_base;
     const int32_t s0 = c.start[p];
     const int32_t n = c.end[p] - s0;
     if (n <= 0) return;
-    fit_wave<T>(static_cast<const T*>(c.values) + s0, n, p, c.type, c.params, c.bits, c.err, lane);
+    switch (c.dtype) {
+        case GLC_I32: fit_wave<int32_t>(static_cast<const int32_t*>(c.values) + s0, n, p, c.type, c.params, c.bits, c.err, lane); break;
+        case GLC_U32: fit_wave<uint32_t>(static_cast<const uint32_t*>(c.values) + s0, n, p, c.type, c.params, c.bits, c.err, lane); break;
+        case GLC_I64: fit_wave<int64_t>(static_cast<const int64_t*>(c.values) + s0, n, p, c.type, c.params, c.bits, c.err, lane); break;
+        default: fit_wave<uint64_t>(static_cast<const uint64_t*>(c.values) + s0, n, p, c.type, c.params, c.bits, c.err, lane); break;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -839,12 +843,11 @@ __global__ __launch_bounds__(kPackThreads) void k_pack_slow(const T* __restrict_
 }
 
 
-// Batched pack over the 1024-value tiles of every column of type T: the wave
-// pack for tiles inside one partition, LDS assembly for the others.
-template <class T>
+// Batched wave pack over the 1024-value tiles of every column, every element
+// type in one launch (the type comes from the column); tiles that span
+// partitions are left to k_pack_batch.
 __global__ __launch_bounds__(64 * kPackWaves) void k_pack_wave_batch(const BatchCol* __restrict__ cols,
-                                                                     const int32_t* __restrict__ tile_col, int64_t Ttot,
-                                                                     int dtype) {
+                                                                     const int32_t* __restrict__ tile_col, int64_t Ttot) {
     __shared__ uint64_t s_code[kPackWaves][kIndexTile];
     __shared__ PackEdge s_edge[2 * kPackWaves];
     const int lane = threadIdx.x & 63;
@@ -853,13 +856,30 @@ __global__ __launch_bounds__(64 * kPackWaves) void k_pack_wave_batch(const Batch
     PackEdge ef{nullptr, 0}, el{nullptr, 0};
     if (g < Ttot) {
         const BatchCol& c = cols[tile_col[g]];
-        if (c.dtype == dtype && c.payload_words <= c.words_reserved) {
+        if (c.payload_words <= c.words_reserved) {
             const int64_t t = g - c.tile_base;
             const TileInfo x = c.tinfo[t];
             if (tile_path(x.info) != TILE_SLOW) {
                 const int64_t e0 = t * kIndexTile;
                 const int64_t e1 = (e0 + kIndexTile < c.n) ? e0 + kIndexTile : c.n;
-                pack_wave<T>(static_cast<const T*>(c.values), c.desc, x, e0, e1, c.words, lane, s_code[wave], ef, el);
+                switch (c.dtype) {
+                    case GLC_I32:
+                        pack_wave<int32_t>(static_cast<const int32_t*>(c.values), c.desc, x, e0, e1, c.words, lane,
+                                           s_code[wave], ef, el);
+                        break;
+                    case GLC_U32:
+                        pack_wave<uint32_t>(static_cast<const uint32_t*>(c.values), c.desc, x, e0, e1, c.words, lane,
+                                            s_code[wave], ef, el);
+                        break;
+                    case GLC_I64:
+                        pack_wave<int64_t>(static_cast<const int64_t*>(c.values), c.desc, x, e0, e1, c.words, lane,
+                                           s_code[wave], ef, el);
+                        break;
+                    default:
+                        pack_wave<uint64_t>(static_cast<const uint64_t*>(c.values), c.desc, x, e0, e1, c.words, lane,
+                                            s_code[wave], ef, el);
+                        break;
+                }
             }
         }
     }
@@ -1270,13 +1290,13 @@ int launch_batch_compress(const BatchPlan& b, hipStream_t s) {
                        b.S);
     GLC_HIP(hipGetLastError());
     const unsigned gf = static_cast<unsigned>(b.Ptot < (1LL << 20) ? b.Ptot : (1LL << 20));
+    hipLaunchKernelGGL(k_fit_wave_batch, dim3(static_cast<unsigned>((b.Ptot + kFitWaves - 1) / kFitWaves)),
+                       dim3(64 * kFitWaves), 0, s, b.cols, b.part_col, b.Ptot);
+    GLC_HIP(hipGetLastError());
     for (int dt = GLC_I32; dt <= GLC_U64; ++dt) {
         if (!b.has_dtype[dt]) continue;
         int rc = dispatch_dtype(dt, [&](auto t) {
             using T = decltype(t);
-            hipLaunchKernelGGL(k_fit_wave_batch<T>, dim3(static_cast<unsigned>((b.Ptot + kFitWaves - 1) / kFitWaves)),
-                               dim3(64 * kFitWaves), 0, s, b.cols, b.part_col, b.Ptot, dt);
-            GLC_HIP(hipGetLastError());
             if (b.has_wg_fit[dt]) {
                 hipLaunchKernelGGL(k_fit_batch<T>, dim3(gf), dim3(kFitThreads), 0, s, b.cols, b.part_col, b.Ptot, dt);
                 GLC_HIP(hipGetLastError());
@@ -1297,12 +1317,13 @@ int launch_batch_compress(const BatchPlan& b, hipStream_t s) {
     GLC_HIP(hipGetLastError());
     const int64_t pchunks = (b.Ttot + kPackThreads - 1) / kPackThreads;
     const unsigned gp = static_cast<unsigned>(pchunks < 8192 ? (pchunks > 0 ? pchunks : 1) : 8192);
-    for (int dt = GLC_I32; dt <= GLC_U64; ++dt) {
+    hipLaunchKernelGGL(k_pack_wave_batch, dim3(grid_for(b.Ttot, kPackWaves)), dim3(64 * kPackWaves), 0, s, b.cols,
+                       b.tile_col, b.Ttot);
+    GLC_HIP(hipGetLastError());
+    for (int dt = GLC_I32; dt <= GLC_U64; ++dt) {  // the partition-spanning tiles, after every wave-packed word
         if (!b.has_dtype[dt]) continue;
         int rc = dispatch_dtype(dt, [&](auto t) {
             using T = decltype(t);
-            hipLaunchKernelGGL(k_pack_wave_batch<T>, dim3(grid_for(b.Ttot, kPackWaves)), dim3(64 * kPackWaves), 0, s,
-                               b.cols, b.tile_col, b.Ttot, dt);
             hipLaunchKernelGGL(k_pack_batch<T>, dim3(gp), dim3(kPackThreads), 0, s, b.cols, b.tile_col, b.Ttot, dt);
             GLC_HIP(hipGetLastError());
             return GLC_OK;
