@@ -523,7 +523,12 @@ int glc_server_create(int64_t hbm_budget_bytes, int64_t staging_bytes, int32_t m
         shutdown(s);
         return fail(GLC_ERR_OOM, "device memory of the serving budget could not be allocated");
     }
-    if (hipHostMalloc(reinterpret_cast<void**>(&s->pinned), static_cast<size_t>(kLookupSlots * slot)) != hipSuccess) {
+    // The lookup kernel reads the staging and the position arena in place,
+    // and the host rewrites them between steps: mapped, coherent memory, so
+    // the device never holds a stale cached copy.
+    constexpr unsigned kStagingFlags = hipHostMallocMapped | hipHostMallocCoherent;
+    if (hipHostMalloc(reinterpret_cast<void**>(&s->pinned), static_cast<size_t>(kLookupSlots * slot), kStagingFlags) !=
+        hipSuccess) {
         (void)hipGetLastError();
         s->pinned = nullptr;
         shutdown(s);
@@ -531,7 +536,8 @@ int glc_server_create(int64_t hbm_budget_bytes, int64_t staging_bytes, int32_t m
     }
     const int64_t arena_bytes = kLookupSlots * slot;
     void* arena_dev = nullptr;
-    if (hipHostMalloc(reinterpret_cast<void**>(&s->parena), static_cast<size_t>(arena_bytes)) != hipSuccess ||
+    if (hipHostMalloc(reinterpret_cast<void**>(&s->parena), static_cast<size_t>(arena_bytes), kStagingFlags) !=
+            hipSuccess ||
         hipHostGetDevicePointer(&arena_dev, s->parena, 0) != hipSuccess || !arena_dev) {
         (void)hipGetLastError();
         shutdown(s);
