@@ -3,21 +3,24 @@
 // requests against an HBM budget, and continuous batching.
 //
 // Runtime structure (one worker thread per server):
-//   submit   client threads append requests to a queue (positions copied);
+//   submit   client threads append requests to a queue; a lookup's
+//            positions are copied by the submitting thread into a pinned
+//            position arena (or kept as a host copy when the arena is full);
 //   admit    the worker takes queued requests in arrival order while the
-//            in-flight slot count and the device pool (the HBM budget) allow;
+//            in-flight count and the device pool (the HBM budget) allow;
 //            a request that does not fit blocks those behind it only after
 //            it has been passed over kMaxBypass times (no starvation);
-//   launch   all newly admitted lookups form one step in a free staging
-//            slot: their positions and a request table go to the device in
-//            one copy and one kernel (k_serve_lookup) answers every lookup
-//            of every request; each decompress runs the full decoder into
-//            its pool block on the decompress stream.  Each of the
-//            kLookupSlots slots has its own stream and staging, so a step's
-//            copy overlaps the previous step's kernel and decompressions
-//            overlap lookups;
-//   retire   events mark requests done; their results stay in the pool
-//            (device memory) until the client releases them.
+//   launch   all newly admitted lookups (up to kServeMaxReqs) form one step
+//            in a free pinned staging slot: the worker writes the step's
+//            request table there (plus any host-copied positions) and one
+//            kernel (k_serve_lookup) reads table and positions over the bus
+//            and answers every lookup of every request -- no H2D copy; each
+//            decompress runs the full decoder into its pool block on the
+//            decompress stream.  Each of the kLookupSlots slots has its own
+//            stream, so steps and decompressions overlap;
+//   retire   events mark requests done (their arena ranges are freed); the
+//            results stay in the pool (device memory) until the client
+//            releases them.
 // Requests arriving while others run are admitted at the next pass of the
 // loop (every few microseconds while work is in flight).
 #include <hip/hip_runtime.h>
