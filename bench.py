@@ -601,7 +601,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=60)
     ap.add_argument("--log2n", type=int, default=28)
     ap.add_argument("--lookups-log2", type=int, default=24)
     ap.add_argument("--mode", default="worksteal", choices=["worksteal", "bitpacked", "v2", "worksteal_adv"])
