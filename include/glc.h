@@ -267,7 +267,8 @@ typedef struct {
 /* hbm_budget_bytes: device memory of the result pool (>= 1 MiB);
  * staging_bytes: pinned host staging for each step's request table and
  * positions, read by the lookup kernel over the bus (0 = 51 MiB, three
- * slots); max_inflight: admitted-but-unreleased cap. */
+ * slots); the server pins the same amount again as the arena that holds
+ * submitted positions; max_inflight: admitted-but-unreleased cap. */
 int glc_server_create(int64_t hbm_budget_bytes, int64_t staging_bytes, int32_t max_inflight, glc_server** out);
 /* Register a resident column (not owned; must outlive the server). */
 int glc_server_add_column(glc_server* s, const glc_column* col, int32_t* col_id);
