@@ -477,6 +477,80 @@ def bench_c5(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_splitmerge(args):
+    """GPU split/merge partitioner (the reference's VariableLengthPartitioner
+    cost model, run on the device) on the C2 column at N = 2^log2n, minimum
+    partition size 128.  One step = one compress call, host wall clock with
+    the device synchronised (the partitioner's passes read their counts back
+    to the host).  Compared with V6 and with the library's host split/merge
+    on the same column; cpu_baseline = the oracle's exact-sum split/merge on
+    the full column, whose boundaries must equal the GPU's."""
+    import numpy as np
+    import torch
+
+    import glc
+    from glc import workloads as W
+
+    n = 1 << args.log2n
+    dev = torch.device("cuda:0")
+    log(f"generating c2 uint64 column on the host, N=2^{args.log2n}")
+    v = W.c2_uint64(n)
+    x = torch.from_numpy(v).to(dev)
+    torch.cuda.synchronize()
+
+    def wall(fn, steps):
+        ts = []
+        for _ in range(steps):
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            r = fn()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t)
+        return r, ts
+
+    gsm = lambda: glc.compress(x, glc.PART_GPU_SPLITMERGE, 128)
+    wall(gsm, args.warmup)
+    torch.cuda.synchronize()
+    t_all = time.perf_counter()
+    col, ts = wall(gsm, args.steps)
+    t_all = time.perf_counter() - t_all
+    inf = col.info()
+    assert torch.equal(col.decompress_full(glc.FULL_AUTO), x)
+    meta = col.meta()
+    st, en = meta["start"], meta["end"]
+    ms = statistics.median(ts) * 1e3
+    v6col, v6ts = wall(lambda: glc.compress(x, glc.PART_V6, 0, 2048), max(3, min(args.steps, 20)))
+    cpucol, cputs = wall(lambda: glc.compress(x, glc.PART_CPU_SPLITMERGE, 128), 1)
+    cpu = None
+    if not args.no_cpu_baseline:
+        import oracle as O
+        t = time.perf_counter()
+        ref = O.compress(v, O.PART_SPLITMERGE_EXACT, 128)
+        ref_s = time.perf_counter() - t
+        assert np.array_equal(ref.start, st) and np.array_equal(ref.end, en), "GPU split/merge boundaries"
+        cpu = {"value": round(n * 8 / ref_s / 1e9, 4), "unit": "GB/s", "cores": 1, "kind": "port",
+               "sample": f"oracle (C++ restatement) exact-sum split/merge compress of the whole column, "
+                         f"{ref_s:.2f} s; its boundaries equal the GPU's"}
+    line = {
+        "metric": "GPU split/merge compress: raw input GB/s",
+        "value": round(n * 8 / (ms * 1e-3) / 1e9, 3), "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "uint64", "data": "synthetic (seeded, generated on the host)",
+        "config": {"workload": f"splitmerge_c2_uint64_N2^{args.log2n}_min128", "n": n,
+                   "partitions": inf.num_partitions, "compressed_bytes": inf.compressed_size,
+                   "timed_total_s": round(t_all, 3)},
+        "compare": {"v6_ms": round(statistics.median(v6ts) * 1e3, 3), "v6_partitions": v6col.num_partitions,
+                    "v6_compressed_bytes": v6col.info().compressed_size,
+                    "host_splitmerge_ms": round(cputs[0] * 1e3, 1),
+                    "host_splitmerge_compressed_bytes": cpucol.info().compressed_size},
+        "roofline": None,
+        "roofline_note": "a search over candidate partitions (FP64 fits and int128 sums per candidate start): "
+                         "bound by vector ALU and by the passes' host round trips, not by HBM or MFMA",
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def bench_e2e(args):
     """Host-memory edges, timed the way the reference times its host-vector
     API (app/BenchmarkUtils.cuh:12-25): host wall clock around whole calls
@@ -607,7 +681,7 @@ def main():
     ap.add_argument("--mode", default="worksteal", choices=["worksteal", "bitpacked", "v2", "worksteal_adv"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-log2", type=int, default=24)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "e2e"])
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "e2e", "splitmerge"])
     ap.add_argument("--c5-columns", type=int, default=512)
     ap.add_argument("--c5-min-log2", type=int, default=20)
     ap.add_argument("--c5-max-log2", type=int, default=26)
@@ -635,6 +709,10 @@ def main():
         return bench_c1(args)
     if args.workload == "e2e":
         return bench_e2e(args)
+    if args.workload == "splitmerge":
+        if args.log2n == 28:
+            args.log2n = 24
+        return bench_splitmerge(args)
     if args.workload == "c3":
         if args.log2n == 28:
             args.log2n = 29

@@ -16,13 +16,17 @@
 //   chain   the greedy walk 0 -> end(0) -> end(end(0)) ... keeps the starts it
 //           reaches: pointer doubling marks them in log2(N/m) rounds;
 //   merge   up to 10 left-to-right passes.  Pass: the merged cost of every
-//           adjacent pair in parallel, then one workgroup walks the list;
-//           while a group is a single partition its decision is the
-//           precomputed pair test, otherwise the workgroup fits the grown
+//           adjacent pair in parallel, then the list is walked in segments
+//           at once with a one-workgroup fix-up where the true walk enters a
+//           segment elsewhere than the segment's own walk (k_sm_seg,
+//           k_sm_fix).  While a group is a single partition its decision is
+//           the precomputed pair test, otherwise the workgroup fits the grown
 //           group on the spot (the reference's sequential semantics).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cfloat>
+#include <cstdlib>
 #include <vector>
 
 #include "glc_device.hpp"
@@ -106,6 +110,24 @@ template <class T> __device__ __forceinline__ int64_t abs_err(T v, double t0, do
     return d < 0 ? static_cast<int64_t>(0ULL - static_cast<uint64_t>(d)) : d;
 }
 
+// Residual magnitude in FP64 when it is exact.  For values v and rounded
+// predictions p of magnitude <= 2^50 (inside T for 32-bit types), sat_cast is
+// the identity on p, calc_delta is v - p without wrap or saturation, and the
+// FP64 difference of two such integers is exact; so fabs(v - round(pr)) is
+// abs_err's result.  The prediction t0 + t1 * i is monotonic in i, so its two
+// end values bound a candidate's whole range.
+template <class T> __device__ __forceinline__ bool small_value(T x) {
+    if constexpr (sizeof(T) == 4) return true;
+    else if constexpr (std::is_signed<T>::value) return x >= -(int64_t(1) << 50) && x <= (int64_t(1) << 50);
+    else return x <= (uint64_t(1) << 50);
+}
+template <class T> __device__ __forceinline__ bool small_pred(double pr) {  // false for NaN
+    if constexpr (std::is_same<T, int32_t>::value) return pr >= -2147483647.0 && pr <= 2147483646.0;
+    else if constexpr (std::is_same<T, uint32_t>::value) return pr >= 0.0 && pr <= 4294967294.0;
+    else if constexpr (std::is_signed<T>::value) return pr >= -1125899906842624.0 && pr <= 1125899906842624.0;
+    else return pr >= 0.0 && pr <= 1125899906842624.0;
+}
+
 __device__ __forceinline__ int32_t bits_of(int64_t mx) {
     if (mx <= 0) return 0;
     const int b = 64 - __builtin_clzll(static_cast<unsigned long long>(mx)) + 1;
@@ -177,6 +199,7 @@ __global__ __launch_bounds__(kSmThreads) void k_sm_split(const T* __restrict__ v
     __shared__ i128 s_seg[kMaxCand][2];
     __shared__ double s_t[kMaxCand][2];
     __shared__ int64_t s_mx[kSmThreads / 64][kMaxCand];
+    __shared__ bool s_fast[kMaxCand];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int64_t j = blockIdx.x;
     if (j >= J) return;
@@ -194,13 +217,16 @@ __global__ __launch_bounds__(kSmThreads) void k_sm_split(const T* __restrict__ v
         return;
     }
     const int64_t Lmax = static_cast<int64_t>(kCandUnits[K - 1]) * m;
+    int big = 0;  // a value outside small_value's range
     // segment sums: segment k = [size(k-1), size(k)), one wave each
     for (int k = w; k < K; k += kSmThreads / 64) {
         const int64_t a = (k == 0) ? 0 : static_cast<int64_t>(kCandUnits[k - 1]) * m;
         const int64_t b = static_cast<int64_t>(kCandUnits[k]) * m;
         i128 sy = 0, sxy = 0;
         for (int64_t i = a + lane; i < b; i += 64) {
-            const i128 y = wide<T>(v[cur + i]);
+            const T x = v[cur + i];
+            big |= small_value<T>(x) ? 0 : 1;
+            const i128 y = wide<T>(x);
             sy += y;
             sxy += y * static_cast<i128>(i);
         }
@@ -211,7 +237,7 @@ __global__ __launch_bounds__(kSmThreads) void k_sm_split(const T* __restrict__ v
             s_seg[k][1] = sxy;
         }
     }
-    __syncthreads();
+    const int any_big = __syncthreads_or(big);
     if (tid < K) {
         i128 sy = 0, sxy = 0;
         for (int k = 0; k <= tid; ++k) {
@@ -219,21 +245,45 @@ __global__ __launch_bounds__(kSmThreads) void k_sm_split(const T* __restrict__ v
             sxy += s_seg[k][1];
         }
         double t0, t1;
-        sm_theta(sy, sxy, static_cast<int64_t>(kCandUnits[tid]) * m, t0, t1);
+        const int64_t size = static_cast<int64_t>(kCandUnits[tid]) * m;
+        sm_theta(sy, sxy, size, t0, t1);
         s_t[tid][0] = t0;
         s_t[tid][1] = t1;
+        const double p0 = t0 + t1 * 0.0, p1 = t0 + t1 * static_cast<double>(size - 1);
+        s_fast[tid] = small_pred<T>(p0) && small_pred<T>(p1);
     }
     __syncthreads();
+    bool fast = !any_big;
+    for (int k = 0; k < K; ++k) fast = fast && s_fast[k];
     int64_t mx[kMaxCand];
 #pragma unroll
     for (int k = 0; k < kMaxCand; ++k) mx[k] = 0;
-    for (int64_t i = tid; i < Lmax; i += kSmThreads) {
-        const T x = v[cur + i];
+    if (fast) {  // every candidate's residuals exact in FP64 (workgroup-uniform)
+        double md[kMaxCand];
 #pragma unroll
-        for (int k = 0; k < kMaxCand; ++k) {
-            if (k < K && i < static_cast<int64_t>(kCandUnits[k]) * m) {
-                const int64_t a = abs_err<T>(x, s_t[k][0], s_t[k][1], i);
-                mx[k] = a > mx[k] ? a : mx[k];
+        for (int k = 0; k < kMaxCand; ++k) md[k] = 0.0;
+        for (int64_t i = tid; i < Lmax; i += kSmThreads) {
+            const double y = static_cast<double>(v[cur + i]);
+            const double xi = static_cast<double>(i);
+#pragma unroll
+            for (int k = 0; k < kMaxCand; ++k) {
+                if (k < K && i < static_cast<int64_t>(kCandUnits[k]) * m) {
+                    const double pr = s_t[k][0] + s_t[k][1] * xi;  // no contraction (-ffp-contract=off)
+                    md[k] = __builtin_fmax(md[k], __builtin_fabs(y - __builtin_round(pr)));
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kMaxCand; ++k) mx[k] = static_cast<int64_t>(md[k]);
+    } else {
+        for (int64_t i = tid; i < Lmax; i += kSmThreads) {
+            const T x = v[cur + i];
+#pragma unroll
+            for (int k = 0; k < kMaxCand; ++k) {
+                if (k < K && i < static_cast<int64_t>(kCandUnits[k]) * m) {
+                    const int64_t a = abs_err<T>(x, s_t[k][0], s_t[k][1], i);
+                    mx[k] = a > mx[k] ? a : mx[k];
+                }
             }
         }
     }
@@ -340,21 +390,40 @@ __global__ __launch_bounds__(kSmThreads) void k_sm_pair(const T* __restrict__ v,
     }
 }
 
-// the sequential pass: out lists + [count, changed]
+// A pass's walk (CpuPartitioner.cuh:207-236) keeps one state: the current
+// group [gs, ge) with its cost.  At a group start that state is fixed by the
+// start's index alone, so the walk from any group start is the same whoever
+// reaches it.  The pass therefore runs as
+//   k_sm_seg  every segment of `seg` consecutive partitions is walked at once,
+//             one workgroup each, as if a group started at its first entry;
+//             the walk records its group starts inside the segment (spec = 1,
+//             group end index, group cost) and its exit, the first group start
+//             at or past the segment's end;
+//   k_sm_fix  one workgroup follows the true walk from partition 0 across the
+//             segments: where it reaches a start that the segment's walk also
+//             made, the segment's result from there on is the true one and the
+//             walk jumps to the segment's exit; elsewhere it grows the group
+//             itself and rewrites spec / end / cost for the entries it covers.
+// Either way every entry of every segment ends up with the true walk's
+// decision, so the pass result equals the sequential walk's.
 constexpr int kWin = 256;
+
 template <class T>
-__global__ __launch_bounds__(kSmThreads) void k_sm_walk(const T* __restrict__ v, const int32_t* __restrict__ st,
-                                                        const int32_t* __restrict__ en,
-                                                        const double* __restrict__ cost,
-                                                        const double* __restrict__ mc,
-                                                        const int32_t* __restrict__ flag, int64_t P,
-                                                        int32_t* __restrict__ ost, int32_t* __restrict__ oen,
-                                                        double* __restrict__ ocost, int64_t* __restrict__ result) {
+__global__ __launch_bounds__(kSmThreads) void k_sm_seg(const T* __restrict__ v, const int32_t* __restrict__ st,
+                                                       const int32_t* __restrict__ en,
+                                                       const double* __restrict__ cost,
+                                                       const double* __restrict__ mc,
+                                                       const int32_t* __restrict__ flag, int64_t P, int64_t seg,
+                                                       int32_t* __restrict__ spec, int32_t* __restrict__ gend,
+                                                       double* __restrict__ gcost, int32_t* __restrict__ exitk) {
     __shared__ FitLds L;
     __shared__ int32_t w_st[kWin + 1], w_en[kWin + 1], w_flag[kWin + 1];
     __shared__ double w_c[kWin + 1], w_mc[kWin + 1];
     const int tid = threadIdx.x;
-    int64_t base = -1;  // window holds entries [base, base + kWin]
+    const int64_t a0 = static_cast<int64_t>(blockIdx.x) * seg;
+    if (a0 >= P) return;
+    const int64_t a1 = (a0 + seg < P) ? a0 + seg : P;
+    int64_t base = a0;  // the window holds entries [base, base + kWin]
     auto load = [&](int64_t b) {
         __syncthreads();
         for (int k = tid; k <= kWin; k += kSmThreads) {
@@ -369,20 +438,18 @@ __global__ __launch_bounds__(kSmThreads) void k_sm_walk(const T* __restrict__ v,
         }
         __syncthreads();
     };
-    load(0);
-    base = 0;
-    int64_t gs = w_st[0], ge = w_en[0];
+    load(a0);
+    int64_t g = a0;  // index of the current group's first partition
+    int64_t gs = w_st[0];
     double gc = w_c[0];
     bool single = true;
-    int64_t out = 0;
-    int64_t changed = 0;
-    for (int64_t i = 1; i < P; ++i) {
+    int64_t i = a0 + 1;
+    for (; i < P; ++i) {
         if (i - base >= kWin) {  // entries i-1 and i must be in the window
             base = i - 1;
             load(base);
         }
         const int k = static_cast<int>(i - base);
-        const int32_t si = w_st[k], ei = w_en[k];
         const double ci = w_c[k];
         bool merge;
         double merged;
@@ -390,34 +457,130 @@ __global__ __launch_bounds__(kSmThreads) void k_sm_walk(const T* __restrict__ v,
             merge = w_flag[k - 1] != 0;
             merged = w_mc[k - 1];
         } else {
-            merged = wg_cost<T>(v, gs, ei, L);
+            merged = wg_cost<T>(v, gs, w_en[k], L);
             merge = merged < gc + ci;
         }
         if (merge) {
-            ge = ei;
             gc = merged;
             single = false;
-            changed = 1;
+            if (tid == 0 && i < a1) spec[i] = 0;
         } else {
             if (tid == 0) {
-                ost[out] = static_cast<int32_t>(gs);
-                oen[out] = static_cast<int32_t>(ge);
-                ocost[out] = gc;
+                spec[g] = 1;
+                gend[g] = static_cast<int32_t>(i);
+                gcost[g] = gc;
             }
-            ++out;
-            gs = si;
-            ge = ei;
+            if (i >= a1) break;
+            g = i;
+            gs = w_st[k];
             gc = ci;
             single = true;
         }
     }
     if (tid == 0) {
-        ost[out] = static_cast<int32_t>(gs);
-        oen[out] = static_cast<int32_t>(ge);
-        ocost[out] = gc;
-        result[0] = out + 1;
-        result[1] = changed;
+        if (i == P) {  // the last group reaches the end of the list
+            spec[g] = 1;
+            gend[g] = static_cast<int32_t>(P);
+            gcost[g] = gc;
+        }
+        exitk[blockIdx.x] = static_cast<int32_t>(i);
     }
+}
+
+// The group that starts at partition e: returns the index of the first
+// partition after it and its cost (all threads call it).
+template <class T>
+__device__ int64_t sm_grow(const T* __restrict__ v, const int32_t* __restrict__ st, const int32_t* __restrict__ en,
+                           const double* __restrict__ cost, const double* __restrict__ mc,
+                           const int32_t* __restrict__ flag, int64_t P, int64_t e, double& gc_out, FitLds& L) {
+    const int64_t gs = st[e];
+    double gc = cost[e];
+    bool single = true;
+    int64_t j = e + 1;
+    for (; j < P; ++j) {
+        const double cj = cost[j];
+        bool merge;
+        double merged;
+        if (single) {
+            merge = flag[j - 1] != 0;
+            merged = mc[j - 1];
+        } else {
+            merged = wg_cost<T>(v, gs, en[j], L);
+            merge = merged < gc + cj;
+        }
+        if (!merge) break;
+        gc = merged;
+        single = false;
+    }
+    gc_out = gc;
+    return j;
+}
+
+constexpr int kFixWin = 2048;
+template <class T>
+__global__ __launch_bounds__(kSmThreads) void k_sm_fix(const T* __restrict__ v, const int32_t* __restrict__ st,
+                                                       const int32_t* __restrict__ en,
+                                                       const double* __restrict__ cost,
+                                                       const double* __restrict__ mc,
+                                                       const int32_t* __restrict__ flag, int64_t P, int64_t seg,
+                                                       int64_t K, int32_t* __restrict__ spec,
+                                                       int32_t* __restrict__ gend, double* __restrict__ gcost,
+                                                       const int32_t* __restrict__ exitk) {
+    __shared__ FitLds L;
+    __shared__ int32_t s_spec[kFixWin], s_exit[kFixWin];
+    const int tid = threadIdx.x;
+    // Every write below is to an entry before the walk's position e, every
+    // read of spec at or after it, so the windows never hold a stale entry
+    // that is read.
+    int64_t wb = -1, xb = -1;  // windows: spec [wb, wb + kFixWin), exits [xb, xb + kFixWin)
+    int64_t e = 0;             // the true walk's next group start
+    for (int64_t k = 0; k < K; ++k) {
+        const int64_t a0 = k * seg;
+        const int64_t a1 = (a0 + seg < P) ? a0 + seg : P;
+        const int64_t c = e < a1 ? e : a1;
+        for (int64_t a = a0 + tid; a < c; a += kSmThreads) spec[a] = 0;  // inside the group before e
+        while (e < a1) {
+            if (wb < 0 || e >= wb + kFixWin) {
+                __syncthreads();
+                wb = e;
+                for (int q = tid; q < kFixWin; q += kSmThreads) s_spec[q] = (wb + q < P) ? spec[wb + q] : 0;
+                __syncthreads();
+            }
+            if (s_spec[e - wb]) {  // the segment's own walk from e is the true one
+                if (xb < 0 || k >= xb + kFixWin) {
+                    __syncthreads();
+                    xb = k;
+                    for (int q = tid; q < kFixWin; q += kSmThreads) s_exit[q] = (xb + q < K) ? exitk[xb + q] : 0;
+                    __syncthreads();
+                }
+                e = s_exit[k - xb];
+                break;
+            }
+            double gc;
+            const int64_t j = sm_grow<T>(v, st, en, cost, mc, flag, P, e, gc, L);
+            if (tid == 0) {
+                spec[e] = 1;
+                gend[e] = static_cast<int32_t>(j);
+                gcost[e] = gc;
+            }
+            const int64_t hi = j < a1 ? j : a1;
+            for (int64_t a = e + 1 + tid; a < hi; a += kSmThreads) spec[a] = 0;
+            e = j;
+        }
+    }
+}
+
+// compaction of the group starts: out lists of the pass
+__global__ void k_sm_emit_groups(const int32_t* __restrict__ spec, const int32_t* __restrict__ pos,
+                                 const int32_t* __restrict__ gend, const double* __restrict__ gcost,
+                                 const int32_t* __restrict__ st, const int32_t* __restrict__ en, int64_t P,
+                                 int32_t* __restrict__ ost, int32_t* __restrict__ oen, double* __restrict__ ocost) {
+    const int64_t a = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (a >= P || !spec[a]) return;
+    const int32_t p = pos[a];
+    ost[p] = st[a];
+    oen[p] = en[gend[a] - 1];
+    ocost[p] = gcost[a];
 }
 
 template <class F> int dispatch_dtype(int dtype, F&& f) {
@@ -431,6 +594,16 @@ template <class F> int dispatch_dtype(int dtype, F&& f) {
 }
 
 unsigned blocks(int64_t n, int b) { return static_cast<unsigned>((n + b - 1) / b); }
+
+// Partitions per speculative merge segment: GLC_SPLITMERGE_SEGMENT (1..2^30),
+// default 32.  Any value gives the same boundaries; it only moves work
+// between the parallel segment walks and the one-workgroup fix-up.
+int64_t merge_segment_length() {
+    const char* e = std::getenv("GLC_SPLITMERGE_SEGMENT");
+    if (!e || !*e) return 32;
+    const long long x = std::atoll(e);
+    return (x >= 1 && x <= (1LL << 30)) ? x : 32;
+}
 
 }  // namespace
 
@@ -458,6 +631,11 @@ int gpu_split_merge(int dtype, const void* d_values, int64_t n, int32_t m, std::
     int32_t *st0 = I(5), *st1 = I(6), *en0 = I(7), *en1 = I(8), *flag = I(9);
     double *best_cost = D(0), *cost0 = D(1), *cost1 = D(2), *mc = D(3);
     int64_t* result = reinterpret_cast<int64_t*>(p + 10 * i4 + 4 * d8);
+    // merge passes reuse the split phase's arrays: group start flags in mark,
+    // group end indices in jump0, segment exits in jump1, group costs in best_cost
+    int32_t *spec = mark, *gend = jump0, *exitk = jump1;
+    double* gcost = best_cost;
+    const int64_t seg = merge_segment_length();
     int rc = dispatch_dtype(dtype, [&](auto t) -> int {
         using T = decltype(t);
         const T* v = static_cast<const T*>(d_values);
@@ -486,18 +664,27 @@ int gpu_split_merge(int dtype, const void* d_values, int64_t n, int32_t m, std::
         for (int iter = 0; iter < 10 && P > 1; ++iter) {  // CpuPartitioner.cuh:198-242
             const unsigned g = static_cast<unsigned>(std::min<int64_t>(P - 1, 1 << 20));
             hipLaunchKernelGGL(k_sm_pair<T>, dim3(g), dim3(kSmThreads), 0, s, v, sa, ea, ca, P, mc, flag);
-            hipLaunchKernelGGL(k_sm_walk<T>, dim3(1), dim3(kSmThreads), 0, s, v, sa, ea, ca, mc, flag, P, sb, eb, cb,
-                               result);
+            // at most 2^20 segments (one workgroup each)
+            const int64_t sg = std::max<int64_t>(seg, (P + (1 << 20) - 1) >> 20);
+            const int64_t K = (P + sg - 1) / sg;
+            hipLaunchKernelGGL(k_sm_seg<T>, dim3(static_cast<unsigned>(K)), dim3(kSmThreads), 0, s, v, sa, ea, ca, mc,
+                               flag, P, sg, spec, gend, gcost, exitk);
+            hipLaunchKernelGGL(k_sm_fix<T>, dim3(1), dim3(kSmThreads), 0, s, v, sa, ea, ca, mc, flag, P, sg, K, spec,
+                               gend, gcost, exitk);
+            hipLaunchKernelGGL(k_sm_scan, dim3(1), dim3(kScan), 0, s, spec, P, pos, result);
+            hipLaunchKernelGGL(k_sm_emit_groups, dim3(blocks(P, 256)), dim3(256), 0, s, spec, pos, gend, gcost, sa, ea,
+                               P, sb, eb, cb);
             GLC_HIP(hipGetLastError());
-            int64_t r[2] = {0, 0};
-            GLC_HIP(hipMemcpyAsync(r, result, 16, hipMemcpyDeviceToHost, s));
+            int64_t Pn = 0;
+            GLC_HIP(hipMemcpyAsync(&Pn, result, 8, hipMemcpyDeviceToHost, s));
             GLC_HIP(hipStreamSynchronize(s));
-            if (r[0] < 1 || r[0] > P) return fail(GLC_ERR_STATE, "merge pass produced an invalid partition count");
-            P = r[0];
+            if (Pn < 1 || Pn > P) return fail(GLC_ERR_STATE, "merge pass produced an invalid partition count");
+            const bool changed = Pn < P;  // every merge removes one entry
+            P = Pn;
             std::swap(sa, sb);
             std::swap(ea, eb);
             std::swap(ca, cb);
-            if (!r[1]) break;
+            if (!changed) break;
         }
         st.resize(static_cast<size_t>(P));
         en.resize(static_cast<size_t>(P));

@@ -118,10 +118,35 @@ def test_parity_small(gpu, gen, part, size):
     full_case(v, part, size or 4096, fixed_size=(size if part == "fixed" else None))
 
 
-def test_gpu_splitmerge_boundaries(gpu):
+def linear_runs(n, seed, run, noise, dtype):
+    """Linear runs of `run` values with +-noise: the merge passes grow groups
+    over many split partitions (long merge chains)."""
+    rng = np.random.default_rng(seed)
+    i = np.arange(n)
+    r = i // run
+    slope = rng.integers(-50, 50, r.max() + 1)
+    off = rng.integers(0, 1 << 20, r.max() + 1)
+    return (off[r] + slope[r] * (i % run) + rng.integers(-noise, noise + 1, n)).astype(dtype)
+
+
+# GLC_SPLITMERGE_SEGMENT: partitions per speculative merge segment.  1 and 3
+# put group boundaries across segments everywhere, 2^30 is one segment (the
+# plain sequential walk); all must give the sequential walk's boundaries.
+@pytest.mark.parametrize("seg", [None, "1", "3", "1073741824"])
+def test_gpu_splitmerge_boundaries(gpu, seg, monkeypatch):
+    if seg is None:
+        monkeypatch.delenv("GLC_SPLITMERGE_SEGMENT", raising=False)
+    else:
+        monkeypatch.setenv("GLC_SPLITMERGE_SEGMENT", seg)
     # exact regime (column prefix sums below 2^53): GPU split/merge == the
     # reference's CPU partitioner restated; otherwise == the exact-sum oracle
-    cases = [(W.c1_int32(1 << 18), 128, O.PART_CPU),
+    cases = [(linear_runs(1 << 17, 1, 40000, 3, np.int64), 128, O.PART_SPLITMERGE_EXACT),
+             (linear_runs(1 << 16, 2, 9000, 0, np.uint32) + np.uint32(1 << 22), 64, O.PART_SPLITMERGE_EXACT),
+             # values straddling 2^50, where the split's FP64 residual shortcut stops being exact
+             (linear_runs(1 << 16, 3, 5000, 2, np.int64) << 31, 128, O.PART_SPLITMERGE_EXACT),
+             (linear_runs(1 << 16, 4, 7000, 1, np.int64).astype(np.uint64) + np.uint64(2**50 - 2**19), 128,
+              O.PART_SPLITMERGE_EXACT),
+             (W.c1_int32(1 << 18), 128, O.PART_CPU),
              (np.where(np.arange(1 << 16) < 30000, np.arange(1 << 16) * 5, 100000 - np.arange(1 << 16)).astype(np.int32),
               128, O.PART_CPU),
              (W.c1_int32(70000), 100, O.PART_CPU),
@@ -130,7 +155,7 @@ def test_gpu_splitmerge_boundaries(gpu):
              (np.arange(300, dtype=np.uint32) * 7, 128, O.PART_CPU),
              (np.array([5], dtype=np.int64), 128, O.PART_CPU)]
     for v, m, mode in cases:
-        print("split/merge case", v.dtype, v.size, m, flush=True)
+        print("split/merge case", v.dtype, v.size, m, "segment", seg, flush=True)
         st, en = glc.partition(to_dev(v), glc.PART_GPU_SPLITMERGE, m)
         ref = O.compress(v, mode, m)
         assert np.array_equal(st, ref.start) and np.array_equal(en, ref.end), (v.dtype, v.size, m)
