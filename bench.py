@@ -519,6 +519,19 @@ def bench_splitmerge(args):
     meta = col.meta()
     st, en = meta["start"], meta["end"]
     ms = statistics.median(ts) * 1e3
+    sweep = {}
+    if args.sm_segments:  # partitions per speculative merge segment: same boundaries, different time
+        saved = os.environ.get("GLC_SPLITMERGE_SEGMENT")
+        for seg in [int(t) for t in args.sm_segments.split(",")]:
+            os.environ["GLC_SPLITMERGE_SEGMENT"] = str(seg)
+            c2, t2 = wall(gsm, max(3, min(args.steps, 10)))
+            m2 = c2.meta()
+            assert np.array_equal(m2["start"], st) and np.array_equal(m2["end"], en), f"segment length {seg}"
+            sweep[str(seg)] = round(statistics.median(t2) * 1e3, 3)
+        if saved is None:
+            os.environ.pop("GLC_SPLITMERGE_SEGMENT")
+        else:
+            os.environ["GLC_SPLITMERGE_SEGMENT"] = saved
     v6col, v6ts = wall(lambda: glc.compress(x, glc.PART_V6, 0, 2048), max(3, min(args.steps, 20)))
     cpucol, cputs = wall(lambda: glc.compress(x, glc.PART_CPU_SPLITMERGE, 128), 1)
     cpu = None
@@ -542,7 +555,8 @@ def bench_splitmerge(args):
         "compare": {"v6_ms": round(statistics.median(v6ts) * 1e3, 3), "v6_partitions": v6col.num_partitions,
                     "v6_compressed_bytes": v6col.info().compressed_size,
                     "host_splitmerge_ms": round(cputs[0] * 1e3, 1),
-                    "host_splitmerge_compressed_bytes": cpucol.info().compressed_size},
+                    "host_splitmerge_compressed_bytes": cpucol.info().compressed_size,
+                    "compress_ms_by_merge_segment": sweep or None},
         "roofline": None,
         "roofline_note": "a search over candidate partitions (FP64 fits and int128 sums per candidate start): "
                          "bound by vector ALU and by the passes' host round trips, not by HBM or MFMA",
@@ -693,6 +707,8 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain device copy of the input (known bytes) for counter calibration")
     ap.add_argument("--c4-columns", type=int, default=10_000)
+    ap.add_argument("--sm-segments", default="",
+                    help="splitmerge workload: comma list of merge segment lengths to time (boundaries must not change)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}). "

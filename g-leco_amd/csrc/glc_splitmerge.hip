@@ -86,10 +86,22 @@ __device__ __forceinline__ int64_t wave_max(int64_t v) {
     return v;
 }
 
+// sum of i^2 for i < n, (n-1) n (2n-1) / 6, for 1 <= n < 2^31 without a
+// 128-bit division: one of n-1, n is even and one of n-1, n, 2n-1 is a
+// multiple of 3, so the division is exact on the factors.
+__device__ __forceinline__ i128 sum_sq(int64_t n) {
+    uint64_t a = static_cast<uint64_t>(n - 1), b = static_cast<uint64_t>(n), c = static_cast<uint64_t>(2 * n - 1);
+    if (a % 2 == 0) a /= 2;
+    else b /= 2;
+    if (a % 3 == 0) a /= 3;
+    else if (b % 3 == 0) b /= 3;
+    else c /= 3;
+    return static_cast<i128>(static_cast<u128>(a * b) * c);  // a * b < 2^62, c < 2^32
+}
+
 // theta of the split/merge cost model from exact sums over n values.
 __device__ void sm_theta(i128 Sy, i128 Sxy, int64_t n, double& t0, double& t1) {
-    const i128 N = n;
-    const i128 Sxx = (N - 1) * N * (2 * N - 1) / 6;
+    const i128 Sxx = sum_sq(n);
     const int32_t nn1 = static_cast<int32_t>(static_cast<uint32_t>(n) * static_cast<uint32_t>(n - 1));
     const double sx = static_cast<double>(nn1) / 2.0;
     const double sy = i128_to_double(Sy), sxy = i128_to_double(Sxy), sxx = i128_to_double(Sxx);
@@ -291,6 +303,141 @@ __global__ __launch_bounds__(kSmThreads) void k_sm_split(const T* __restrict__ v
     for (int k = 0; k < kMaxCand; ++k) {
         const int64_t r = wave_max(mx[k]);
         if (lane == 0) s_mx[w][k] = r;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double best = DBL_MAX, bc = 0.0;
+        int64_t be = (cur + m < n) ? cur + m : n;
+        for (int k = 0; k < K; ++k) {
+            int64_t r = s_mx[0][k];
+            for (int q = 1; q < kSmThreads / 64; ++q) r = s_mx[q][k] > r ? s_mx[q][k] : r;
+            const int64_t size = static_cast<int64_t>(kCandUnits[k]) * m;
+            const double c = cost_of(size, bits_of(r));
+            const double cpe = c / static_cast<double>(size);
+            if (cpe < best) {
+                best = cpe;
+                be = cur + size;
+                bc = c;
+            }
+        }
+        best_end[j] = static_cast<int32_t>(be);
+        best_cost[j] = bc;
+    }
+}
+
+// The same evaluation for m <= kStageMin, where every candidate of a start
+// fits in LDS: the start's values are loaded once, all loads of a thread in
+// flight together, and both passes read LDS.  The maximum-error pass runs
+// candidate by candidate with theta in registers (one accumulator, reduced
+// as soon as the candidate is done).
+constexpr int kStageMin = 128;
+constexpr int kStage = 29 * kStageMin;  // the largest candidate, 29 * m values
+template <class T>
+__global__ __launch_bounds__(kSmThreads) void k_sm_split_lds(const T* __restrict__ v, int64_t n, int32_t m,
+                                                             int64_t J, int32_t* __restrict__ best_end,
+                                                             double* __restrict__ best_cost) {
+    typedef typename std::make_unsigned<T>::type U;
+    __shared__ uint64_t s_v[kStage];  // T's bits; FP64 values in the exact pass
+    __shared__ i128 s_seg[kMaxCand][2];
+    __shared__ double s_t[kMaxCand][2];
+    __shared__ int64_t s_mx[kSmThreads / 64][kMaxCand];
+    __shared__ bool s_fast[kMaxCand];
+    __shared__ FitLds L;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int64_t j = blockIdx.x;
+    if (j >= J) return;
+    const int64_t cur = j * m;
+    const int64_t max_size = (n - cur < static_cast<int64_t>(m) * 32) ? n - cur : static_cast<int64_t>(m) * 32;
+    int K = 0;
+    while (K < kMaxCand && static_cast<int64_t>(kCandUnits[K]) * m <= max_size) ++K;
+    if (K == 0) {  // fewer than m values left: one partition [cur, n)
+        const double c = wg_cost<T>(v, cur, n, L);
+        if (tid == 0) {
+            best_end[j] = static_cast<int32_t>(n);
+            best_cost[j] = c;
+        }
+        return;
+    }
+    const int64_t Lmax = static_cast<int64_t>(kCandUnits[K - 1]) * m;  // <= kStage, <= n - cur
+    constexpr int kPer = (kStage + kSmThreads - 1) / kSmThreads;
+    int big = 0;  // a value outside small_value's range
+    {
+        T y[kPer];
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) {
+            const int64_t i = tid + q * kSmThreads;
+            y[q] = v[cur + (i < Lmax ? i : Lmax - 1)];
+        }
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) {
+            const int64_t i = tid + q * kSmThreads;
+            if (i < Lmax) {
+                big |= small_value<T>(y[q]) ? 0 : 1;
+                s_v[i] = static_cast<uint64_t>(static_cast<U>(y[q]));
+            }
+        }
+    }
+    __syncthreads();
+    for (int k = w; k < K; k += kSmThreads / 64) {  // segment k = [size(k-1), size(k)), one wave each
+        const int64_t a = (k == 0) ? 0 : static_cast<int64_t>(kCandUnits[k - 1]) * m;
+        const int64_t b = static_cast<int64_t>(kCandUnits[k]) * m;
+        i128 sy = 0, sxy = 0;
+        for (int64_t i = a + lane; i < b; i += 64) {
+            const i128 y = wide<T>(static_cast<T>(static_cast<U>(s_v[i])));
+            sy += y;
+            sxy += y * static_cast<i128>(i);
+        }
+        sy = wave_sum(sy);
+        sxy = wave_sum(sxy);
+        if (lane == 0) {
+            s_seg[k][0] = sy;
+            s_seg[k][1] = sxy;
+        }
+    }
+    const int any_big = __syncthreads_or(big);
+    if (tid < K) {
+        i128 sy = 0, sxy = 0;
+        for (int k = 0; k <= tid; ++k) {
+            sy += s_seg[k][0];
+            sxy += s_seg[k][1];
+        }
+        double t0, t1;
+        const int64_t size = static_cast<int64_t>(kCandUnits[tid]) * m;
+        sm_theta(sy, sxy, size, t0, t1);
+        s_t[tid][0] = t0;
+        s_t[tid][1] = t1;
+        const double p0 = t0 + t1 * 0.0, p1 = t0 + t1 * static_cast<double>(size - 1);
+        s_fast[tid] = small_pred<T>(p0) && small_pred<T>(p1);
+    }
+    __syncthreads();
+    bool fast = !any_big;
+    for (int k = 0; k < K; ++k) fast = fast && s_fast[k];
+    if (fast) {  // workgroup-uniform: every residual is exact in FP64
+        for (int64_t i = tid; i < Lmax; i += kSmThreads)
+            s_v[i] = static_cast<uint64_t>(
+                __double_as_longlong(static_cast<double>(static_cast<T>(static_cast<U>(s_v[i])))));
+        __syncthreads();
+    }
+    for (int k = 0; k < K; ++k) {
+        const int size = kCandUnits[k] * m;
+        const double t0 = s_t[k][0], t1 = s_t[k][1];
+        int64_t mx = 0;
+        if (fast) {
+            double md = 0.0;
+            for (int i = tid; i < size; i += kSmThreads) {
+                const double pr = t0 + t1 * static_cast<double>(i);  // no contraction (-ffp-contract=off)
+                md = __builtin_fmax(md, __builtin_fabs(__longlong_as_double(static_cast<long long>(s_v[i])) -
+                                                       __builtin_round(pr)));
+            }
+            mx = static_cast<int64_t>(md);
+        } else {
+            for (int i = tid; i < size; i += kSmThreads) {
+                const int64_t a = abs_err<T>(static_cast<T>(static_cast<U>(s_v[i])), t0, t1, i);
+                mx = a > mx ? a : mx;
+            }
+        }
+        mx = wave_max(mx);
+        if (lane == 0) s_mx[w][k] = mx;
     }
     __syncthreads();
     if (tid == 0) {
@@ -639,8 +786,12 @@ int gpu_split_merge(int dtype, const void* d_values, int64_t n, int32_t m, std::
     int rc = dispatch_dtype(dtype, [&](auto t) -> int {
         using T = decltype(t);
         const T* v = static_cast<const T*>(d_values);
-        hipLaunchKernelGGL(k_sm_split<T>, dim3(static_cast<unsigned>(J)), dim3(kSmThreads), 0, s, v, n, m, J,
-                           best_end, best_cost);
+        if (m <= kStageMin)
+            hipLaunchKernelGGL(k_sm_split_lds<T>, dim3(static_cast<unsigned>(J)), dim3(kSmThreads), 0, s, v, n, m, J,
+                               best_end, best_cost);
+        else
+            hipLaunchKernelGGL(k_sm_split<T>, dim3(static_cast<unsigned>(J)), dim3(kSmThreads), 0, s, v, n, m, J,
+                               best_end, best_cost);
         GLC_HIP(hipGetLastError());
         hipLaunchKernelGGL(k_sm_chain_init, dim3(blocks(J, 256)), dim3(256), 0, s, best_end, J, m, jump0, mark);
         GLC_HIP(hipGetLastError());

@@ -47,7 +47,7 @@ typedef enum {
     GLC_PART_FIXED = 0,           /* [iS, min((i+1)S, N)), G:401-416 (S is a parameter; the reference hard-codes 4096) */
     GLC_PART_V6 = 1,              /* GPUVariableLengthPartitionerV6, compression/GpuPartitioner.cuh:34-214 */
     GLC_PART_CPU_SPLITMERGE = 2,  /* VariableLengthPartitioner on the host, compression/CpuPartitioner.cuh:12-423 */
-    GLC_PART_GPU_SPLITMERGE = 3,  /* same cost model, split candidates evaluated on the GPU */
+    GLC_PART_GPU_SPLITMERGE = 3,  /* same algorithm and cost model, split and merge passes on the GPU */
     GLC_PART_EXPLICIT = 4         /* caller-supplied partition list (glc_compress_partitions) */
 } glc_partitioner;
 
