@@ -325,20 +325,112 @@ __global__ __launch_bounds__(kSmThreads) void k_sm_split(const T* __restrict__ v
     }
 }
 
+// Exact prefix sums at block granularity: for p in [0, J], G1(p) = sum of
+// y_q and G2(p) = sum of q * y_q over q < min(p * m, n) (q the column index).
+// |G2| < 2^126 for n < 2^31, so i128 holds them.  k_sm_block_pfx: one
+// workgroup per chunk of kPfxChunk positions, one wave per block sum, then an
+// exclusive scan inside the chunk (p1, p2) and the chunk totals (c1, c2);
+// k_sm_chunk_scan turns the totals into chunk offsets (exclusive).
+// G(p) = p(p) + c(p / kPfxChunk).
+constexpr int kPfxChunk = kSmThreads;
+template <class T>
+__global__ __launch_bounds__(kSmThreads) void k_sm_block_pfx(const T* __restrict__ v, int64_t n, int32_t m,
+                                                             int64_t J, i128* __restrict__ p1,
+                                                             i128* __restrict__ p2, i128* __restrict__ c1,
+                                                             i128* __restrict__ c2) {
+    __shared__ i128 s1[kPfxChunk], s2[kPfxChunk];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int64_t p0 = static_cast<int64_t>(blockIdx.x) * kPfxChunk;
+    for (int b = w; b < kPfxChunk; b += kSmThreads / 64) {  // block p0 + b: values [(p0+b) m, (p0+b+1) m)
+        const int64_t j = p0 + b;
+        i128 a1 = 0, a2 = 0;
+        if (j < J) {
+            const int64_t lo = j * m;
+            const int64_t hi = (lo + m < n) ? lo + m : n;
+            for (int64_t q = lo + lane; q < hi; q += 64) {
+                const i128 y = wide<T>(v[q]);
+                a1 += y;
+                a2 += y * static_cast<i128>(q);
+            }
+            a1 = wave_sum(a1);
+            a2 = wave_sum(a2);
+        }
+        if (lane == 0) {
+            s1[b] = a1;
+            s2[b] = a2;
+        }
+    }
+    __syncthreads();
+    for (int o = 1; o < kPfxChunk; o <<= 1) {  // inclusive scan
+        const i128 x1 = (tid >= o) ? s1[tid - o] : static_cast<i128>(0);
+        const i128 x2 = (tid >= o) ? s2[tid - o] : static_cast<i128>(0);
+        __syncthreads();
+        s1[tid] += x1;
+        s2[tid] += x2;
+        __syncthreads();
+    }
+    const int64_t p = p0 + tid;
+    if (p <= J) {
+        p1[p] = tid ? s1[tid - 1] : static_cast<i128>(0);
+        p2[p] = tid ? s2[tid - 1] : static_cast<i128>(0);
+    }
+    if (tid == kPfxChunk - 1) {
+        c1[blockIdx.x] = s1[tid];
+        c2[blockIdx.x] = s2[tid];
+    }
+}
+// exclusive scan of C chunk totals in place (one workgroup, serial chunks)
+__global__ __launch_bounds__(kSmThreads) void k_sm_chunk_scan(i128* __restrict__ c1, i128* __restrict__ c2,
+                                                              int64_t C) {
+    __shared__ i128 s1[kSmThreads], s2[kSmThreads];
+    const int tid = threadIdx.x;
+    const int64_t per = (C + kSmThreads - 1) / kSmThreads;
+    const int64_t b0 = tid * per;
+    const int64_t b1 = (b0 + per < C) ? b0 + per : C;
+    i128 a1 = 0, a2 = 0;
+    for (int64_t i = b0; i < b1; ++i) {
+        a1 += c1[i];
+        a2 += c2[i];
+    }
+    s1[tid] = a1;
+    s2[tid] = a2;
+    __syncthreads();
+    for (int o = 1; o < kSmThreads; o <<= 1) {
+        const i128 x1 = (tid >= o) ? s1[tid - o] : static_cast<i128>(0);
+        const i128 x2 = (tid >= o) ? s2[tid - o] : static_cast<i128>(0);
+        __syncthreads();
+        s1[tid] += x1;
+        s2[tid] += x2;
+        __syncthreads();
+    }
+    i128 r1 = s1[tid] - a1, r2 = s2[tid] - a2;  // exclusive
+    for (int64_t i = b0; i < b1; ++i) {
+        const i128 t1 = c1[i], t2 = c2[i];
+        c1[i] = r1;
+        c2[i] = r2;
+        r1 += t1;
+        r2 += t2;
+    }
+}
+
 // The same evaluation for m <= kStageMin, where every candidate of a start
 // fits in LDS: the start's values are loaded once, all loads of a thread in
-// flight together, and both passes read LDS.  The maximum-error pass runs
+// flight together; the sums come from the block prefixes, the maximum-error
+// pass reads LDS.  The maximum-error pass runs
 // candidate by candidate with theta in registers (one accumulator, reduced
 // as soon as the candidate is done).
 constexpr int kStageMin = 128;
 constexpr int kStage = 29 * kStageMin;  // the largest candidate, 29 * m values
 template <class T>
 __global__ __launch_bounds__(kSmThreads) void k_sm_split_lds(const T* __restrict__ v, int64_t n, int32_t m,
-                                                             int64_t J, int32_t* __restrict__ best_end,
+                                                             int64_t J, const i128* __restrict__ p1,
+                                                             const i128* __restrict__ p2,
+                                                             const i128* __restrict__ c1,
+                                                             const i128* __restrict__ c2,
+                                                             int32_t* __restrict__ best_end,
                                                              double* __restrict__ best_cost) {
     typedef typename std::make_unsigned<T>::type U;
     __shared__ uint64_t s_v[kStage];  // T's bits; FP64 values in the exact pass
-    __shared__ i128 s_seg[kMaxCand][2];
     __shared__ double s_t[kMaxCand][2];
     __shared__ int64_t s_mx[kSmThreads / 64][kMaxCand];
     __shared__ bool s_fast[kMaxCand];
@@ -377,37 +469,18 @@ __global__ __launch_bounds__(kSmThreads) void k_sm_split_lds(const T* __restrict
             }
         }
     }
-    __syncthreads();
-    for (int k = w; k < K; k += kSmThreads / 64) {  // segment k = [size(k-1), size(k)), one wave each
-        const int64_t a = (k == 0) ? 0 : static_cast<int64_t>(kCandUnits[k - 1]) * m;
-        const int64_t b = static_cast<int64_t>(kCandUnits[k]) * m;
-        i128 sy = 0, sxy = 0;
-        for (int64_t i = a + lane; i < b; i += 64) {
-            const i128 y = wide<T>(static_cast<T>(static_cast<U>(s_v[i])));
-            sy += y;
-            sxy += y * static_cast<i128>(i);
-        }
-        sy = wave_sum(sy);
-        sxy = wave_sum(sxy);
-        if (lane == 0) {
-            s_seg[k][0] = sy;
-            s_seg[k][1] = sxy;
-        }
-    }
     const int any_big = __syncthreads_or(big);
-    if (tid < K) {
-        i128 sy = 0, sxy = 0;
-        for (int k = 0; k <= tid; ++k) {
-            sy += s_seg[k][0];
-            sxy += s_seg[k][1];
-        }
+    if (tid < K) {  // candidate tid = blocks [j, j + units): sums from the prefixes
+        const int64_t e = j + kCandUnits[tid];  // e * m <= n, so e <= J
+        const i128 sy = (p1[e] + c1[e / kPfxChunk]) - (p1[j] + c1[j / kPfxChunk]);
+        const i128 sxy = (p2[e] + c2[e / kPfxChunk]) - (p2[j] + c2[j / kPfxChunk]) - static_cast<i128>(cur) * sy;
         double t0, t1;
         const int64_t size = static_cast<int64_t>(kCandUnits[tid]) * m;
         sm_theta(sy, sxy, size, t0, t1);
         s_t[tid][0] = t0;
         s_t[tid][1] = t1;
-        const double p0 = t0 + t1 * 0.0, p1 = t0 + t1 * static_cast<double>(size - 1);
-        s_fast[tid] = small_pred<T>(p0) && small_pred<T>(p1);
+        const double pr0 = t0 + t1 * 0.0, pr1 = t0 + t1 * static_cast<double>(size - 1);
+        s_fast[tid] = small_pred<T>(pr0) && small_pred<T>(pr1);
     }
     __syncthreads();
     bool fast = !any_big;
@@ -765,9 +838,13 @@ int gpu_split_merge(int dtype, const void* d_values, int64_t n, int32_t m, std::
     const int64_t J = (n + m - 1) / m;
     // scratch: best_end, jump x2, mark, pos, st x2, en x2, flag (int32 x J each);
     // best_cost, cost x2, mc (double x J each); result (2 x int64)
+    // with m <= kStageMin also block prefixes p1, p2 (i128 x (J + 1)) and chunk offsets c1, c2 (i128 x C)
     const int64_t i4 = align_up(J * 4, 256), d8 = align_up(J * 8, 256);
+    const bool staged = m <= kStageMin;
+    const int64_t C = (J + 1 + kPfxChunk - 1) / kPfxChunk;
+    const int64_t q16 = staged ? align_up((J + 1) * 16, 256) : 0, c16 = staged ? align_up(C * 16, 256) : 0;
     void* mem = nullptr;
-    if (hipMalloc(&mem, static_cast<size_t>(10 * i4 + 4 * d8 + 256)) != hipSuccess) {
+    if (hipMalloc(&mem, static_cast<size_t>(10 * i4 + 4 * d8 + 256 + 2 * q16 + 2 * c16)) != hipSuccess) {
         (void)hipGetLastError();
         return fail(GLC_ERR_OOM, "split/merge scratch allocation failed");
     }
@@ -778,6 +855,9 @@ int gpu_split_merge(int dtype, const void* d_values, int64_t n, int32_t m, std::
     int32_t *st0 = I(5), *st1 = I(6), *en0 = I(7), *en1 = I(8), *flag = I(9);
     double *best_cost = D(0), *cost0 = D(1), *cost1 = D(2), *mc = D(3);
     int64_t* result = reinterpret_cast<int64_t*>(p + 10 * i4 + 4 * d8);
+    char* pq = p + 10 * i4 + 4 * d8 + 256;
+    i128 *pf1 = reinterpret_cast<i128*>(pq), *pf2 = reinterpret_cast<i128*>(pq + q16);
+    i128 *cf1 = reinterpret_cast<i128*>(pq + 2 * q16), *cf2 = reinterpret_cast<i128*>(pq + 2 * q16 + c16);
     // merge passes reuse the split phase's arrays: group start flags in mark,
     // group end indices in jump0, segment exits in jump1, group costs in best_cost
     int32_t *spec = mark, *gend = jump0, *exitk = jump1;
@@ -786,10 +866,13 @@ int gpu_split_merge(int dtype, const void* d_values, int64_t n, int32_t m, std::
     int rc = dispatch_dtype(dtype, [&](auto t) -> int {
         using T = decltype(t);
         const T* v = static_cast<const T*>(d_values);
-        if (m <= kStageMin)
+        if (staged) {
+            hipLaunchKernelGGL(k_sm_block_pfx<T>, dim3(static_cast<unsigned>(C)), dim3(kSmThreads), 0, s, v, n, m, J,
+                               pf1, pf2, cf1, cf2);
+            hipLaunchKernelGGL(k_sm_chunk_scan, dim3(1), dim3(kSmThreads), 0, s, cf1, cf2, C);
             hipLaunchKernelGGL(k_sm_split_lds<T>, dim3(static_cast<unsigned>(J)), dim3(kSmThreads), 0, s, v, n, m, J,
-                               best_end, best_cost);
-        else
+                               pf1, pf2, cf1, cf2, best_end, best_cost);
+        } else
             hipLaunchKernelGGL(k_sm_split<T>, dim3(static_cast<unsigned>(J)), dim3(kSmThreads), 0, s, v, n, m, J,
                                best_end, best_cost);
         GLC_HIP(hipGetLastError());
