@@ -813,7 +813,11 @@ __global__ __launch_bounds__(64 * kPackWaves) void k_pack_wave(const T* __restri
 }
 
 // The tiles the wave pack leaves (several partitions, gaps, POLY models):
-// LDS assembly per 1024-value tile.
+// LDS assembly per 1024-value tile.  A workgroup scans kSlowChunk tiles at a
+// time (one path read per thread), lists the slow ones in LDS and packs them
+// one after another; small chunks spread columns with many slow tiles (a
+// partition boundary inside every few tiles) over many workgroups.
+constexpr int kSlowChunk = 8;
 template <class T>
 __global__ __launch_bounds__(kPackThreads) void k_pack_slow(const T* __restrict__ v, const Desc* __restrict__ desc,
                                                             const int32_t* __restrict__ tile_part,
@@ -821,16 +825,15 @@ __global__ __launch_bounds__(kPackThreads) void k_pack_slow(const T* __restrict_
                                                             int64_t ntiles, uint32_t* __restrict__ words) {
     __shared__ uint32_t s_w[kPackWords];
     __shared__ int64_t s_ww[2];
-    __shared__ int64_t s_list[kPackThreads];
+    __shared__ int64_t s_list[kSlowChunk];
     __shared__ int s_cnt;
-    // chunks of kPackThreads tiles: every thread reads one tile's path, the
-    // slow ones are listed in LDS and packed one after another
-    for (int64_t c0 = static_cast<int64_t>(blockIdx.x) * kPackThreads; c0 < ntiles;
-         c0 += static_cast<int64_t>(gridDim.x) * kPackThreads) {
+    for (int64_t c0 = static_cast<int64_t>(blockIdx.x) * kSlowChunk; c0 < ntiles;
+         c0 += static_cast<int64_t>(gridDim.x) * kSlowChunk) {
         if (threadIdx.x == 0) s_cnt = 0;
         __syncthreads();
         const int64_t t = c0 + threadIdx.x;
-        if (t < ntiles && tile_path(tinfo[t].info) == TILE_SLOW) s_list[atomicAdd(&s_cnt, 1)] = t;
+        if (threadIdx.x < kSlowChunk && t < ntiles && tile_path(tinfo[t].info) == TILE_SLOW)
+            s_list[atomicAdd(&s_cnt, 1)] = t;
         __syncthreads();
         const int cnt = s_cnt;
         for (int k = 0; k < cnt; ++k) {
@@ -899,14 +902,14 @@ __global__ __launch_bounds__(kPackThreads) void k_pack_batch(const BatchCol* __r
                                                              int dtype) {
     __shared__ uint32_t s_w[kPackWords];
     __shared__ int64_t s_ww[2];
-    __shared__ int64_t s_list[kPackThreads];
+    __shared__ int64_t s_list[kSlowChunk];
     __shared__ int s_cnt;
-    for (int64_t c0 = static_cast<int64_t>(blockIdx.x) * kPackThreads; c0 < Ttot;
-         c0 += static_cast<int64_t>(gridDim.x) * kPackThreads) {
+    for (int64_t c0 = static_cast<int64_t>(blockIdx.x) * kSlowChunk; c0 < Ttot;
+         c0 += static_cast<int64_t>(gridDim.x) * kSlowChunk) {
         if (threadIdx.x == 0) s_cnt = 0;
         __syncthreads();
         const int64_t g = c0 + threadIdx.x;
-        if (g < Ttot) {
+        if (threadIdx.x < kSlowChunk && g < Ttot) {
             const BatchCol& c = cols[tile_col[g]];
             if (c.dtype == dtype && c.payload_words <= c.words_reserved &&
                 tile_path(c.tinfo[g - c.tile_base].info) == TILE_SLOW)
@@ -1190,7 +1193,7 @@ int launch_pack(int dtype, const void* d_values, glc_column* c, hipStream_t s) {
                            static_cast<const T*>(d_values),
                            c->d_desc, c->d_tinfo, c->n, c->d_words);
         GLC_HIP(hipGetLastError());
-        const int64_t chunks = (ntiles + kPackThreads - 1) / kPackThreads;
+        const int64_t chunks = (ntiles + kSlowChunk - 1) / kSlowChunk;
         const unsigned gs = static_cast<unsigned>(chunks < 8192 ? (chunks > 0 ? chunks : 1) : 8192);
         hipLaunchKernelGGL(k_pack_slow<T>, dim3(gs), dim3(kPackThreads), 0, s, static_cast<const T*>(d_values),
                            c->d_desc, c->d_tile_part, c->d_tinfo, c->n, ntiles, c->d_words);
@@ -1315,7 +1318,7 @@ int launch_batch_compress(const BatchPlan& b, hipStream_t s) {
     GLC_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_zero_edges_batch, dim3(grid_for(b.Ttot, 256)), dim3(256), 0, s, b.cols, b.tile_col, b.Ttot);
     GLC_HIP(hipGetLastError());
-    const int64_t pchunks = (b.Ttot + kPackThreads - 1) / kPackThreads;
+    const int64_t pchunks = (b.Ttot + kSlowChunk - 1) / kSlowChunk;
     const unsigned gp = static_cast<unsigned>(pchunks < 8192 ? (pchunks > 0 ? pchunks : 1) : 8192);
     hipLaunchKernelGGL(k_pack_wave_batch, dim3(grid_for(b.Ttot, kPackWaves)), dim3(64 * kPackWaves), 0, s, b.cols,
                        b.tile_col, b.Ttot);
