@@ -413,14 +413,15 @@ __global__ __launch_bounds__(kSmThreads) void k_sm_chunk_scan(i128* __restrict__
     }
 }
 
-// The same evaluation for m <= kStageMin, where every candidate of a start
-// fits in LDS: the start's values are loaded once, all loads of a thread in
-// flight together; the sums come from the block prefixes, the maximum-error
-// pass reads LDS.  The maximum-error pass runs
-// candidate by candidate with theta in registers (one accumulator, reduced
-// as soon as the candidate is done).
+// The same evaluation for m <= kStageMin, kSplitStarts consecutive starts per
+// workgroup: their candidates lie in one window of (kSplitStarts - 1 + 29) m
+// values, loaded into LDS once with all loads of a thread in flight.  The sums
+// come from the block prefixes.  The maximum-error pass runs candidate by
+// candidate with theta in registers (one accumulator, reduced as soon as the
+// candidate is done).
 constexpr int kStageMin = 128;
-constexpr int kStage = 29 * kStageMin;  // the largest candidate, 29 * m values
+constexpr int kSplitStarts = 4;
+constexpr int kStage = (kSplitStarts - 1 + 29) * kStageMin;  // the window, values
 template <class T>
 __global__ __launch_bounds__(kSmThreads) void k_sm_split_lds(const T* __restrict__ v, int64_t n, int32_t m,
                                                              int64_t J, const i128* __restrict__ p1,
@@ -431,94 +432,118 @@ __global__ __launch_bounds__(kSmThreads) void k_sm_split_lds(const T* __restrict
                                                              double* __restrict__ best_cost) {
     typedef typename std::make_unsigned<T>::type U;
     __shared__ uint64_t s_v[kStage];  // T's bits; FP64 values in the exact pass
-    __shared__ double s_t[kMaxCand][2];
-    __shared__ int64_t s_mx[kSmThreads / 64][kMaxCand];
-    __shared__ bool s_fast[kMaxCand];
+    __shared__ double s_t[kSplitStarts][kMaxCand][2];
+    __shared__ int64_t s_mx[kSmThreads / 64][kSplitStarts][kMaxCand];
+    __shared__ bool s_fast[kSplitStarts][kMaxCand];
     __shared__ FitLds L;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const int64_t j = blockIdx.x;
-    if (j >= J) return;
-    const int64_t cur = j * m;
-    const int64_t max_size = (n - cur < static_cast<int64_t>(m) * 32) ? n - cur : static_cast<int64_t>(m) * 32;
-    int K = 0;
-    while (K < kMaxCand && static_cast<int64_t>(kCandUnits[K]) * m <= max_size) ++K;
-    if (K == 0) {  // fewer than m values left: one partition [cur, n)
-        const double c = wg_cost<T>(v, cur, n, L);
-        if (tid == 0) {
-            best_end[j] = static_cast<int32_t>(n);
-            best_cost[j] = c;
-        }
-        return;
+    const int64_t j0 = static_cast<int64_t>(blockIdx.x) * kSplitStarts;
+    if (j0 >= J) return;
+    const int64_t cur0 = j0 * m;
+    // candidates of start s: K[s] (0: fewer than m values left, only the column's last start)
+    int K[kSplitStarts];
+    int ns = 0, Kmax = 0;
+    bool tail = false;
+#pragma unroll
+    for (int q = 0; q < kSplitStarts; ++q) {
+        K[q] = 0;
+        const int64_t cur = cur0 + static_cast<int64_t>(q) * m;
+        if (j0 + q >= J) continue;
+        ns = q + 1;
+        const int64_t max_size = (n - cur < static_cast<int64_t>(m) * 32) ? n - cur : static_cast<int64_t>(m) * 32;
+        int k = 0;
+        while (k < kMaxCand && static_cast<int64_t>(kCandUnits[k]) * m <= max_size) ++k;
+        K[q] = k;
+        Kmax = k > Kmax ? k : Kmax;
+        tail = tail || k == 0;
     }
-    const int64_t Lmax = static_cast<int64_t>(kCandUnits[K - 1]) * m;  // <= kStage, <= n - cur
+    // window [cur0, cur0 + W): every candidate of every start, W <= kStage, cur0 + W <= n
+    int64_t W = 0;
+#pragma unroll
+    for (int q = 0; q < kSplitStarts; ++q)
+        if (K[q] > 0) {
+            const int64_t end = static_cast<int64_t>(q) * m + static_cast<int64_t>(kCandUnits[K[q] - 1]) * m;
+            W = end > W ? end : W;
+        }
     constexpr int kPer = (kStage + kSmThreads - 1) / kSmThreads;
     int big = 0;  // a value outside small_value's range
-    {
+    if (W > 0) {
         T y[kPer];
 #pragma unroll
         for (int q = 0; q < kPer; ++q) {
             const int64_t i = tid + q * kSmThreads;
-            y[q] = v[cur + (i < Lmax ? i : Lmax - 1)];
+            y[q] = v[cur0 + (i < W ? i : W - 1)];
         }
 #pragma unroll
         for (int q = 0; q < kPer; ++q) {
             const int64_t i = tid + q * kSmThreads;
-            if (i < Lmax) {
+            if (i < W) {
                 big |= small_value<T>(y[q]) ? 0 : 1;
                 s_v[i] = static_cast<uint64_t>(static_cast<U>(y[q]));
             }
         }
     }
     const int any_big = __syncthreads_or(big);
-    if (tid < K) {  // candidate tid = blocks [j, j + units): sums from the prefixes
-        const int64_t e = j + kCandUnits[tid];  // e * m <= n, so e <= J
-        const i128 sy = (p1[e] + c1[e / kPfxChunk]) - (p1[j] + c1[j / kPfxChunk]);
-        const i128 sxy = (p2[e] + c2[e / kPfxChunk]) - (p2[j] + c2[j / kPfxChunk]) - static_cast<i128>(cur) * sy;
-        double t0, t1;
-        const int64_t size = static_cast<int64_t>(kCandUnits[tid]) * m;
-        sm_theta(sy, sxy, size, t0, t1);
-        s_t[tid][0] = t0;
-        s_t[tid][1] = t1;
-        const double pr0 = t0 + t1 * 0.0, pr1 = t0 + t1 * static_cast<double>(size - 1);
-        s_fast[tid] = small_pred<T>(pr0) && small_pred<T>(pr1);
+    if (tid < kSplitStarts * kMaxCand) {  // candidate k of start q: sums from the prefixes
+        const int q = tid / kMaxCand, k = tid % kMaxCand;
+        if (k < K[q]) {
+            const int64_t j = j0 + q, cur = cur0 + static_cast<int64_t>(q) * m;
+            const int64_t e = j + kCandUnits[k];  // e * m <= n, so e <= J
+            const i128 sy = (p1[e] + c1[e / kPfxChunk]) - (p1[j] + c1[j / kPfxChunk]);
+            const i128 sxy =
+                (p2[e] + c2[e / kPfxChunk]) - (p2[j] + c2[j / kPfxChunk]) - static_cast<i128>(cur) * sy;
+            double t0, t1;
+            const int64_t size = static_cast<int64_t>(kCandUnits[k]) * m;
+            sm_theta(sy, sxy, size, t0, t1);
+            s_t[q][k][0] = t0;
+            s_t[q][k][1] = t1;
+            const double pr0 = t0 + t1 * 0.0, pr1 = t0 + t1 * static_cast<double>(size - 1);
+            s_fast[q][k] = small_pred<T>(pr0) && small_pred<T>(pr1);
+        }
     }
     __syncthreads();
     bool fast = !any_big;
-    for (int k = 0; k < K; ++k) fast = fast && s_fast[k];
+    for (int q = 0; q < ns; ++q)
+        for (int k = 0; k < K[q]; ++k) fast = fast && s_fast[q][k];
     if (fast) {  // workgroup-uniform: every residual is exact in FP64
-        for (int64_t i = tid; i < Lmax; i += kSmThreads)
+        for (int64_t i = tid; i < W; i += kSmThreads)
             s_v[i] = static_cast<uint64_t>(
                 __double_as_longlong(static_cast<double>(static_cast<T>(static_cast<U>(s_v[i])))));
         __syncthreads();
     }
-    for (int k = 0; k < K; ++k) {
-        const int size = kCandUnits[k] * m;
-        const double t0 = s_t[k][0], t1 = s_t[k][1];
-        int64_t mx = 0;
-        if (fast) {
-            double md = 0.0;
-            for (int i = tid; i < size; i += kSmThreads) {
-                const double pr = t0 + t1 * static_cast<double>(i);  // no contraction (-ffp-contract=off)
-                md = __builtin_fmax(md, __builtin_fabs(__longlong_as_double(static_cast<long long>(s_v[i])) -
-                                                       __builtin_round(pr)));
+    for (int q = 0; q < ns; ++q) {
+        const uint64_t* sv = s_v + q * m;  // start q's values
+        for (int k = 0; k < K[q]; ++k) {
+            const int size = kCandUnits[k] * m;
+            const double t0 = s_t[q][k][0], t1 = s_t[q][k][1];
+            int64_t mx = 0;
+            if (fast) {
+                double md = 0.0;
+                for (int i = tid; i < size; i += kSmThreads) {
+                    const double pr = t0 + t1 * static_cast<double>(i);  // no contraction (-ffp-contract=off)
+                    md = __builtin_fmax(md, __builtin_fabs(__longlong_as_double(static_cast<long long>(sv[i])) -
+                                                           __builtin_round(pr)));
+                }
+                mx = static_cast<int64_t>(md);
+            } else {
+                for (int i = tid; i < size; i += kSmThreads) {
+                    const int64_t a = abs_err<T>(static_cast<T>(static_cast<U>(sv[i])), t0, t1, i);
+                    mx = a > mx ? a : mx;
+                }
             }
-            mx = static_cast<int64_t>(md);
-        } else {
-            for (int i = tid; i < size; i += kSmThreads) {
-                const int64_t a = abs_err<T>(static_cast<T>(static_cast<U>(s_v[i])), t0, t1, i);
-                mx = a > mx ? a : mx;
-            }
+            mx = wave_max(mx);
+            if (lane == 0) s_mx[w][q][k] = mx;
         }
-        mx = wave_max(mx);
-        if (lane == 0) s_mx[w][k] = mx;
     }
     __syncthreads();
-    if (tid == 0) {
+    if (tid < ns && K[tid] > 0) {  // start tid: the smallest cost per value, the first on ties
+        const int q = tid;
+        const int64_t cur = cur0 + static_cast<int64_t>(q) * m;
         double best = DBL_MAX, bc = 0.0;
         int64_t be = (cur + m < n) ? cur + m : n;
-        for (int k = 0; k < K; ++k) {
-            int64_t r = s_mx[0][k];
-            for (int q = 1; q < kSmThreads / 64; ++q) r = s_mx[q][k] > r ? s_mx[q][k] : r;
+        for (int k = 0; k < K[q]; ++k) {
+            int64_t r = s_mx[0][q][k];
+            for (int x = 1; x < kSmThreads / 64; ++x) r = s_mx[x][q][k] > r ? s_mx[x][q][k] : r;
             const int64_t size = static_cast<int64_t>(kCandUnits[k]) * m;
             const double c = cost_of(size, bits_of(r));
             const double cpe = c / static_cast<double>(size);
@@ -528,8 +553,19 @@ __global__ __launch_bounds__(kSmThreads) void k_sm_split_lds(const T* __restrict
                 bc = c;
             }
         }
-        best_end[j] = static_cast<int32_t>(be);
-        best_cost[j] = bc;
+        best_end[j0 + q] = static_cast<int32_t>(be);
+        best_cost[j0 + q] = bc;
+    }
+    if (tail) {  // workgroup-uniform: the column's last start has fewer than m values, one partition [cur, n)
+        for (int q = 0; q < ns; ++q) {
+            if (K[q] != 0) continue;
+            const int64_t cur = cur0 + static_cast<int64_t>(q) * m;
+            const double c = wg_cost<T>(v, cur, n, L);
+            if (tid == 0) {
+                best_end[j0 + q] = static_cast<int32_t>(n);
+                best_cost[j0 + q] = c;
+            }
+        }
     }
 }
 
@@ -870,8 +906,8 @@ int gpu_split_merge(int dtype, const void* d_values, int64_t n, int32_t m, std::
             hipLaunchKernelGGL(k_sm_block_pfx<T>, dim3(static_cast<unsigned>(C)), dim3(kSmThreads), 0, s, v, n, m, J,
                                pf1, pf2, cf1, cf2);
             hipLaunchKernelGGL(k_sm_chunk_scan, dim3(1), dim3(kSmThreads), 0, s, cf1, cf2, C);
-            hipLaunchKernelGGL(k_sm_split_lds<T>, dim3(static_cast<unsigned>(J)), dim3(kSmThreads), 0, s, v, n, m, J,
-                               pf1, pf2, cf1, cf2, best_end, best_cost);
+            hipLaunchKernelGGL(k_sm_split_lds<T>, dim3(static_cast<unsigned>((J + kSplitStarts - 1) / kSplitStarts)),
+                               dim3(kSmThreads), 0, s, v, n, m, J, pf1, pf2, cf1, cf2, best_end, best_cost);
         } else
             hipLaunchKernelGGL(k_sm_split<T>, dim3(static_cast<unsigned>(J)), dim3(kSmThreads), 0, s, v, n, m, J,
                                best_end, best_cost);
