@@ -36,6 +36,7 @@ namespace glc {
 namespace {
 
 constexpr int kSmThreads = 256;
+constexpr int kPfxChunk = kSmThreads;  // block-prefix positions per chunk (k_sm_block_pfx)
 constexpr int kMaxCand = 14;  // m * {1,2,3,4,5,7,9,11,13,15,17,21,25,29}
 __constant__ int kCandUnits[kMaxCand] = {1, 2, 3, 4, 5, 7, 9, 11, 13, 15, 17, 21, 25, 29};
 
@@ -155,40 +156,78 @@ struct FitLds {
     i128 sxy[kSmThreads / 64];
     int64_t mx[kSmThreads / 64];
     double t0, t1, cost;
+    int fast;  // both end predictions within small_pred's range
 };
 
-// Cost of [s, e) computed by the whole workgroup (all threads call it).
-template <class T> __device__ double wg_cost(const T* __restrict__ v, int64_t s, int64_t e, FitLds& L) {
+// Block prefix sums of the column (k_sm_block_pfx) when they exist (p1 null
+// otherwise): G1(p), G2(p) over the first min(p m, n) values.
+struct SmPrefix {
+    const i128 *p1, *p2, *c1, *c2;
+    int64_t m, n, J;
+};
+__device__ __forceinline__ i128 pfx_at(const i128* p, const i128* c, int64_t i) {
+    return p[i] + c[i / kPfxChunk];
+}
+
+// Cost of [s, e) computed by the whole workgroup (all threads call it).  The
+// sums come from the block prefixes when s is a block start and e a block
+// start or the column's end (every split/merge boundary is), otherwise from a
+// pass over the values.  The maximum residual takes the exact FP64 form for
+// values within small_value's range when both end predictions are in range.
+template <class T>
+__device__ double wg_cost(const T* __restrict__ v, int64_t s, int64_t e, FitLds& L, const SmPrefix& pf) {
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int64_t n = e - s;
-    i128 sy = 0, sxy = 0;
-    for (int64_t i = tid; i < n; i += kSmThreads) {
-        const i128 y = wide<T>(v[s + i]);
-        sy += y;
-        sxy += y * static_cast<i128>(i);
-    }
-    sy = wave_sum(sy);
-    sxy = wave_sum(sxy);
-    if (lane == 0) {
-        L.sy[w] = sy;
-        L.sxy[w] = sxy;
-    }
-    __syncthreads();
-    if (tid == 0) {
-        i128 a = 0, b = 0;
-        for (int k = 0; k < kSmThreads / 64; ++k) {
-            a += L.sy[k];
-            b += L.sxy[k];
+    const bool use_pfx = pf.p1 != nullptr && s % pf.m == 0 && (e == pf.n || e % pf.m == 0);
+    if (use_pfx) {
+        if (tid == 0) {
+            const int64_t ib = s / pf.m, ie = (e == pf.n) ? pf.J : e / pf.m;
+            const i128 sy = pfx_at(pf.p1, pf.c1, ie) - pfx_at(pf.p1, pf.c1, ib);
+            const i128 sxy = pfx_at(pf.p2, pf.c2, ie) - pfx_at(pf.p2, pf.c2, ib) - static_cast<i128>(s) * sy;
+            sm_theta(sy, sxy, n, L.t0, L.t1);
+            L.fast = small_pred<T>(L.t0 + L.t1 * 0.0) && small_pred<T>(L.t0 + L.t1 * static_cast<double>(n - 1));
         }
-        sm_theta(a, b, n, L.t0, L.t1);
+    } else {
+        i128 sy = 0, sxy = 0;
+        for (int64_t i = tid; i < n; i += kSmThreads) {
+            const i128 y = wide<T>(v[s + i]);
+            sy += y;
+            sxy += y * static_cast<i128>(i);
+        }
+        sy = wave_sum(sy);
+        sxy = wave_sum(sxy);
+        if (lane == 0) {
+            L.sy[w] = sy;
+            L.sxy[w] = sxy;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            i128 a = 0, b = 0;
+            for (int k = 0; k < kSmThreads / 64; ++k) {
+                a += L.sy[k];
+                b += L.sxy[k];
+            }
+            sm_theta(a, b, n, L.t0, L.t1);
+            L.fast = small_pred<T>(L.t0 + L.t1 * 0.0) && small_pred<T>(L.t0 + L.t1 * static_cast<double>(n - 1));
+        }
     }
     __syncthreads();
     const double t0 = L.t0, t1 = L.t1;
+    const bool fast = L.fast != 0;
     int64_t mx = 0;
+    double md = 0.0;
     for (int64_t i = tid; i < n; i += kSmThreads) {
-        const int64_t a = abs_err<T>(v[s + i], t0, t1, i);
-        mx = a > mx ? a : mx;
+        const T x = v[s + i];
+        if (fast && small_value<T>(x)) {
+            const double pr = t0 + t1 * static_cast<double>(i);  // no contraction (-ffp-contract=off)
+            md = __builtin_fmax(md, __builtin_fabs(static_cast<double>(x) - __builtin_round(pr)));
+        } else {
+            const int64_t a = abs_err<T>(x, t0, t1, i);
+            mx = a > mx ? a : mx;
+        }
     }
+    const int64_t mdi = static_cast<int64_t>(md);
+    mx = mdi > mx ? mdi : mx;
     mx = wave_max(mx);
     if (lane == 0) L.mx[w] = mx;
     __syncthreads();
@@ -221,7 +260,7 @@ __global__ __launch_bounds__(kSmThreads) void k_sm_split(const T* __restrict__ v
     while (K < kMaxCand && static_cast<int64_t>(kCandUnits[K]) * m <= max_size) ++K;
     if (K == 0) {  // fewer than m values left: one partition [cur, n)
         __shared__ FitLds L;
-        const double c = wg_cost<T>(v, cur, n, L);
+        const double c = wg_cost<T>(v, cur, n, L, SmPrefix{nullptr, nullptr, nullptr, nullptr, m, n, J});
         if (tid == 0) {
             best_end[j] = static_cast<int32_t>(n);
             best_cost[j] = c;
@@ -332,7 +371,6 @@ __global__ __launch_bounds__(kSmThreads) void k_sm_split(const T* __restrict__ v
 // exclusive scan inside the chunk (p1, p2) and the chunk totals (c1, c2);
 // k_sm_chunk_scan turns the totals into chunk offsets (exclusive).
 // G(p) = p(p) + c(p / kPfxChunk).
-constexpr int kPfxChunk = kSmThreads;
 template <class T>
 __global__ __launch_bounds__(kSmThreads) void k_sm_block_pfx(const T* __restrict__ v, int64_t n, int32_t m,
                                                              int64_t J, i128* __restrict__ p1,
@@ -560,7 +598,7 @@ __global__ __launch_bounds__(kSmThreads) void k_sm_split_lds(const T* __restrict
         for (int q = 0; q < ns; ++q) {
             if (K[q] != 0) continue;
             const int64_t cur = cur0 + static_cast<int64_t>(q) * m;
-            const double c = wg_cost<T>(v, cur, n, L);
+            const double c = wg_cost<T>(v, cur, n, L, SmPrefix{p1, p2, c1, c2, m, n, J});
             if (tid == 0) {
                 best_end[j0 + q] = static_cast<int32_t>(n);
                 best_cost[j0 + q] = c;
@@ -635,10 +673,11 @@ template <class T>
 __global__ __launch_bounds__(kSmThreads) void k_sm_pair(const T* __restrict__ v, const int32_t* __restrict__ st,
                                                         const int32_t* __restrict__ en,
                                                         const double* __restrict__ cost, int64_t P,
-                                                        double* __restrict__ mc, int32_t* __restrict__ flag) {
+                                                        double* __restrict__ mc, int32_t* __restrict__ flag,
+                                                        SmPrefix pf) {
     __shared__ FitLds L;
     for (int64_t i = blockIdx.x; i + 1 < P; i += gridDim.x) {
-        const double c = wg_cost<T>(v, st[i], en[i + 1], L);
+        const double c = wg_cost<T>(v, st[i], en[i + 1], L, pf);
         if (threadIdx.x == 0) {
             mc[i] = c;
             flag[i] = c < cost[i] + cost[i + 1];
@@ -671,7 +710,8 @@ __global__ __launch_bounds__(kSmThreads) void k_sm_seg(const T* __restrict__ v, 
                                                        const double* __restrict__ mc,
                                                        const int32_t* __restrict__ flag, int64_t P, int64_t seg,
                                                        int32_t* __restrict__ spec, int32_t* __restrict__ gend,
-                                                       double* __restrict__ gcost, int32_t* __restrict__ exitk) {
+                                                       double* __restrict__ gcost, int32_t* __restrict__ exitk,
+                                                       SmPrefix pf) {
     __shared__ FitLds L;
     __shared__ int32_t w_st[kWin + 1], w_en[kWin + 1], w_flag[kWin + 1];
     __shared__ double w_c[kWin + 1], w_mc[kWin + 1];
@@ -713,7 +753,7 @@ __global__ __launch_bounds__(kSmThreads) void k_sm_seg(const T* __restrict__ v, 
             merge = w_flag[k - 1] != 0;
             merged = w_mc[k - 1];
         } else {
-            merged = wg_cost<T>(v, gs, w_en[k], L);
+            merged = wg_cost<T>(v, gs, w_en[k], L, pf);
             merge = merged < gc + ci;
         }
         if (merge) {
@@ -748,7 +788,8 @@ __global__ __launch_bounds__(kSmThreads) void k_sm_seg(const T* __restrict__ v, 
 template <class T>
 __device__ int64_t sm_grow(const T* __restrict__ v, const int32_t* __restrict__ st, const int32_t* __restrict__ en,
                            const double* __restrict__ cost, const double* __restrict__ mc,
-                           const int32_t* __restrict__ flag, int64_t P, int64_t e, double& gc_out, FitLds& L) {
+                           const int32_t* __restrict__ flag, int64_t P, int64_t e, double& gc_out, FitLds& L,
+                           const SmPrefix& pf) {
     const int64_t gs = st[e];
     double gc = cost[e];
     bool single = true;
@@ -761,7 +802,7 @@ __device__ int64_t sm_grow(const T* __restrict__ v, const int32_t* __restrict__ 
             merge = flag[j - 1] != 0;
             merged = mc[j - 1];
         } else {
-            merged = wg_cost<T>(v, gs, en[j], L);
+            merged = wg_cost<T>(v, gs, en[j], L, pf);
             merge = merged < gc + cj;
         }
         if (!merge) break;
@@ -781,7 +822,7 @@ __global__ __launch_bounds__(kSmThreads) void k_sm_fix(const T* __restrict__ v, 
                                                        const int32_t* __restrict__ flag, int64_t P, int64_t seg,
                                                        int64_t K, int32_t* __restrict__ spec,
                                                        int32_t* __restrict__ gend, double* __restrict__ gcost,
-                                                       const int32_t* __restrict__ exitk) {
+                                                       const int32_t* __restrict__ exitk, SmPrefix pf) {
     __shared__ FitLds L;
     __shared__ int32_t s_spec[kFixWin], s_exit[kFixWin];
     const int tid = threadIdx.x;
@@ -813,7 +854,7 @@ __global__ __launch_bounds__(kSmThreads) void k_sm_fix(const T* __restrict__ v, 
                 break;
             }
             double gc;
-            const int64_t j = sm_grow<T>(v, st, en, cost, mc, flag, P, e, gc, L);
+            const int64_t j = sm_grow<T>(v, st, en, cost, mc, flag, P, e, gc, L, pf);
             if (tid == 0) {
                 spec[e] = 1;
                 gend[e] = static_cast<int32_t>(j);
@@ -929,18 +970,19 @@ int gpu_split_merge(int dtype, const void* d_values, int64_t n, int32_t m, std::
         GLC_HIP(hipMemcpyAsync(&P, result, 8, hipMemcpyDeviceToHost, s));
         GLC_HIP(hipStreamSynchronize(s));
         if (P < 1 || P > J) return fail(GLC_ERR_STATE, "split phase produced an invalid partition count");
+        const SmPrefix pf{staged ? pf1 : nullptr, pf2, cf1, cf2, m, n, J};  // merge refits: sums from the prefixes
         int32_t *sa = st0, *sb = st1, *ea = en0, *eb = en1;
         double *ca = cost0, *cb = cost1;
         for (int iter = 0; iter < 10 && P > 1; ++iter) {  // CpuPartitioner.cuh:198-242
             const unsigned g = static_cast<unsigned>(std::min<int64_t>(P - 1, 1 << 20));
-            hipLaunchKernelGGL(k_sm_pair<T>, dim3(g), dim3(kSmThreads), 0, s, v, sa, ea, ca, P, mc, flag);
+            hipLaunchKernelGGL(k_sm_pair<T>, dim3(g), dim3(kSmThreads), 0, s, v, sa, ea, ca, P, mc, flag, pf);
             // at most 2^20 segments (one workgroup each)
             const int64_t sg = std::max<int64_t>(seg, (P + (1 << 20) - 1) >> 20);
             const int64_t K = (P + sg - 1) / sg;
             hipLaunchKernelGGL(k_sm_seg<T>, dim3(static_cast<unsigned>(K)), dim3(kSmThreads), 0, s, v, sa, ea, ca, mc,
-                               flag, P, sg, spec, gend, gcost, exitk);
+                               flag, P, sg, spec, gend, gcost, exitk, pf);
             hipLaunchKernelGGL(k_sm_fix<T>, dim3(1), dim3(kSmThreads), 0, s, v, sa, ea, ca, mc, flag, P, sg, K, spec,
-                               gend, gcost, exitk);
+                               gend, gcost, exitk, pf);
             hipLaunchKernelGGL(k_sm_scan, dim3(1), dim3(kScan), 0, s, spec, P, pos, result);
             hipLaunchKernelGGL(k_sm_emit_groups, dim3(blocks(P, 256)), dim3(256), 0, s, spec, pos, gend, gcost, sa, ea,
                                P, sb, eb, cb);
