@@ -656,6 +656,46 @@ __global__ __launch_bounds__(kScan) void k_sm_scan(const int32_t* __restrict__ m
     }
     if (tid == kScan - 1) *count = s[tid];
 }
+// Exclusive scan of a long flag array: k_sm_scan_local scans kScanTile
+// entries per workgroup (16 consecutive per thread) and writes the tile
+// totals; k_sm_scan (one workgroup) scans the totals into tile offsets and the
+// grand total; k_sm_scan_add adds the offsets.
+constexpr int kScanPer = 16;
+constexpr int kScanTile = kSmThreads * kScanPer;
+__global__ __launch_bounds__(kSmThreads) void k_sm_scan_local(const int32_t* __restrict__ mark, int64_t N,
+                                                              int32_t* __restrict__ pos,
+                                                              int32_t* __restrict__ tile_tot) {
+    __shared__ int32_t s[kSmThreads];
+    const int tid = threadIdx.x;
+    const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanTile + static_cast<int64_t>(tid) * kScanPer;
+    int32_t f[kScanPer];
+    int32_t sum = 0;
+#pragma unroll
+    for (int q = 0; q < kScanPer; ++q) {
+        f[q] = (base + q < N) ? mark[base + q] : 0;
+        sum += f[q];
+    }
+    s[tid] = sum;
+    __syncthreads();
+    for (int o = 1; o < kSmThreads; o <<= 1) {
+        const int32_t y = (tid >= o) ? s[tid - o] : 0;
+        __syncthreads();
+        s[tid] += y;
+        __syncthreads();
+    }
+    int32_t acc = s[tid] - sum;
+#pragma unroll
+    for (int q = 0; q < kScanPer; ++q) {
+        if (base + q < N) pos[base + q] = acc;
+        acc += f[q];
+    }
+    if (tid == kSmThreads - 1) tile_tot[blockIdx.x] = s[tid];
+}
+__global__ void k_sm_scan_add(int32_t* __restrict__ pos, int64_t N, const int32_t* __restrict__ tile_off) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < N) pos[i] += tile_off[i / kScanTile];
+}
+
 __global__ void k_sm_emit(const int32_t* __restrict__ mark, const int32_t* __restrict__ pos,
                           const int32_t* __restrict__ best_end, const double* __restrict__ best_cost, int64_t J,
                           int32_t m, int32_t* __restrict__ st, int32_t* __restrict__ en, double* __restrict__ cost) {
@@ -920,8 +960,10 @@ int gpu_split_merge(int dtype, const void* d_values, int64_t n, int32_t m, std::
     const bool staged = m <= kStageMin;
     const int64_t C = (J + 1 + kPfxChunk - 1) / kPfxChunk;
     const int64_t q16 = staged ? align_up((J + 1) * 16, 256) : 0, c16 = staged ? align_up(C * 16, 256) : 0;
+    const int64_t T = (J + kScanTile - 1) / kScanTile;  // scan tiles; tile totals and offsets: int32 x T each
+    const int64_t t4 = align_up(T * 4, 256);
     void* mem = nullptr;
-    if (hipMalloc(&mem, static_cast<size_t>(10 * i4 + 4 * d8 + 256 + 2 * q16 + 2 * c16)) != hipSuccess) {
+    if (hipMalloc(&mem, static_cast<size_t>(10 * i4 + 4 * d8 + 256 + 2 * q16 + 2 * c16 + 2 * t4)) != hipSuccess) {
         (void)hipGetLastError();
         return fail(GLC_ERR_OOM, "split/merge scratch allocation failed");
     }
@@ -935,6 +977,16 @@ int gpu_split_merge(int dtype, const void* d_values, int64_t n, int32_t m, std::
     char* pq = p + 10 * i4 + 4 * d8 + 256;
     i128 *pf1 = reinterpret_cast<i128*>(pq), *pf2 = reinterpret_cast<i128*>(pq + q16);
     i128 *cf1 = reinterpret_cast<i128*>(pq + 2 * q16), *cf2 = reinterpret_cast<i128*>(pq + 2 * q16 + c16);
+    int32_t* tile_tot = reinterpret_cast<int32_t*>(pq + 2 * q16 + 2 * c16);
+    int32_t* tile_off = reinterpret_cast<int32_t*>(pq + 2 * q16 + 2 * c16 + t4);
+    // exclusive scan of flags[0, N) into pos, the total into result[0]
+    auto scan_flags = [&](const int32_t* flags, int64_t N) {
+        const int64_t nt = (N + kScanTile - 1) / kScanTile;
+        hipLaunchKernelGGL(k_sm_scan_local, dim3(static_cast<unsigned>(nt)), dim3(kSmThreads), 0, s, flags, N, pos,
+                           tile_tot);
+        hipLaunchKernelGGL(k_sm_scan, dim3(1), dim3(kScan), 0, s, tile_tot, nt, tile_off, result);
+        hipLaunchKernelGGL(k_sm_scan_add, dim3(blocks(N, 256)), dim3(256), 0, s, pos, N, tile_off);
+    };
     // merge passes reuse the split phase's arrays: group start flags in mark,
     // group end indices in jump0, segment exits in jump1, group costs in best_cost
     int32_t *spec = mark, *gend = jump0, *exitk = jump1;
@@ -962,7 +1014,7 @@ int gpu_split_merge(int dtype, const void* d_values, int64_t n, int32_t m, std::
             GLC_HIP(hipGetLastError());
             std::swap(ja, jb);
         }
-        hipLaunchKernelGGL(k_sm_scan, dim3(1), dim3(kScan), 0, s, mark, J, pos, result);
+        scan_flags(mark, J);
         hipLaunchKernelGGL(k_sm_emit, dim3(blocks(J, 256)), dim3(256), 0, s, mark, pos, best_end, best_cost, J, m,
                            st0, en0, cost0);
         GLC_HIP(hipGetLastError());
@@ -983,7 +1035,7 @@ int gpu_split_merge(int dtype, const void* d_values, int64_t n, int32_t m, std::
                                flag, P, sg, spec, gend, gcost, exitk, pf);
             hipLaunchKernelGGL(k_sm_fix<T>, dim3(1), dim3(kSmThreads), 0, s, v, sa, ea, ca, mc, flag, P, sg, K, spec,
                                gend, gcost, exitk, pf);
-            hipLaunchKernelGGL(k_sm_scan, dim3(1), dim3(kScan), 0, s, spec, P, pos, result);
+            scan_flags(spec, P);
             hipLaunchKernelGGL(k_sm_emit_groups, dim3(blocks(P, 256)), dim3(256), 0, s, spec, pos, gend, gcost, sa, ea,
                                P, sb, eb, cb);
             GLC_HIP(hipGetLastError());
