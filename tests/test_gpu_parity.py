@@ -152,6 +152,8 @@ def test_gpu_splitmerge_boundaries(gpu, seg, monkeypatch):
              (W.c1_int32(70000), 100, O.PART_CPU),
              (W.c2_uint64(1 << 17, seed=4), 128, O.PART_SPLITMERGE_EXACT),
              (W.c3_int64(1 << 17, seed=5), 256, O.PART_SPLITMERGE_EXACT),
+             # values above 2^50 through the LDS split (m <= 128): integer residuals, prefix-sum refits
+             (W.c3_int64(1 << 19, seed=7), 128, O.PART_SPLITMERGE_EXACT),
              (np.arange(300, dtype=np.uint32) * 7, 128, O.PART_CPU),
              (np.array([5], dtype=np.int64), 128, O.PART_CPU)]
     for v, m, mode in cases:
