@@ -8,11 +8,14 @@
 // parallel reductions here give the same costs -- hence the same boundaries --
 // as a sequential evaluation of the same definition.
 //
-//   split   every start s = j*m (m = minimum size) is evaluated at once, one
-//           workgroup each: the candidate sizes m*{1,2,3,4,5,7,...,17,21,25,29}
-//           (step m, then 2m above 4m, then 4m above 16m, at most 32m) share
-//           one pass for the sums (nested prefixes) and one for the maximum
-//           error; the smallest cost per value wins (first on ties);
+//   split   every start s = j*m (m = minimum size) is evaluated at once: the
+//           candidate sizes m*{1,2,3,4,5,7,...,17,21,25,29} (step m, then 2m
+//           above 4m, then 4m above 16m, at most 32m); the smallest cost per
+//           value wins (first on ties).  For m <= 128 a workgroup takes four
+//           starts, their values staged in LDS, their sums from exact block
+//           prefix sums of the column (k_sm_block_pfx); otherwise one start
+//           per workgroup with one pass for the sums (nested prefixes) and
+//           one for the maximum error;
 //   chain   the greedy walk 0 -> end(0) -> end(end(0)) ... keeps the starts it
 //           reaches: pointer doubling marks them in log2(N/m) rounds;
 //   merge   up to 10 left-to-right passes.  Pass: the merged cost of every
