@@ -370,8 +370,8 @@ __global__ __launch_bounds__(kSmThreads) void k_sm_split(const T* __restrict__ v
 // Exact prefix sums at block granularity: for p in [0, J], G1(p) = sum of
 // y_q and G2(p) = sum of q * y_q over q < min(p * m, n) (q the column index).
 // |G2| < 2^126 for n < 2^31, so i128 holds them.  k_sm_block_pfx: one
-// workgroup per chunk of kPfxChunk positions, one wave per block sum, then an
-// exclusive scan inside the chunk (p1, p2) and the chunk totals (c1, c2);
+// workgroup per chunk of kPfxChunk positions, one thread per block sum, then
+// an exclusive scan inside the chunk (p1, p2) and the chunk totals (c1, c2);
 // k_sm_chunk_scan turns the totals into chunk offsets (exclusive).
 // G(p) = p(p) + c(p / kPfxChunk).
 template <class T>
@@ -380,26 +380,22 @@ __global__ __launch_bounds__(kSmThreads) void k_sm_block_pfx(const T* __restrict
                                                              i128* __restrict__ p2, i128* __restrict__ c1,
                                                              i128* __restrict__ c2) {
     __shared__ i128 s1[kPfxChunk], s2[kPfxChunk];
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int tid = threadIdx.x;
     const int64_t p0 = static_cast<int64_t>(blockIdx.x) * kPfxChunk;
-    for (int b = w; b < kPfxChunk; b += kSmThreads / 64) {  // block p0 + b: values [(p0+b) m, (p0+b+1) m)
-        const int64_t j = p0 + b;
+    {  // thread tid: block p0 + tid, values [(p0+tid) m, (p0+tid+1) m) in order
+        const int64_t j = p0 + tid;
         i128 a1 = 0, a2 = 0;
         if (j < J) {
             const int64_t lo = j * m;
             const int64_t hi = (lo + m < n) ? lo + m : n;
-            for (int64_t q = lo + lane; q < hi; q += 64) {
+            for (int64_t q = lo; q < hi; ++q) {
                 const i128 y = wide<T>(v[q]);
                 a1 += y;
                 a2 += y * static_cast<i128>(q);
             }
-            a1 = wave_sum(a1);
-            a2 = wave_sum(a2);
         }
-        if (lane == 0) {
-            s1[b] = a1;
-            s2[b] = a2;
-        }
+        s1[tid] = a1;
+        s2[tid] = a2;
     }
     __syncthreads();
     for (int o = 1; o < kPfxChunk; o <<= 1) {  // inclusive scan
