@@ -129,6 +129,18 @@ def linear_runs(n, seed, run, noise, dtype):
     return (off[r] + slope[r] * (i % run) + rng.integers(-noise, noise + 1, n)).astype(dtype)
 
 
+def jumpy_runs(n, seed, run, noise):
+    """Linear int64 runs of `run` values at random offsets below 2^40: merging
+    across a run boundary never pays, so many partitions survive even at tiny
+    minimum sizes."""
+    rng = np.random.default_rng(seed)
+    i = np.arange(n)
+    r = i // run
+    off = rng.integers(0, 1 << 40, r.max() + 1)
+    slope = rng.integers(-50, 50, r.max() + 1)
+    return (off[r] + slope[r] * (i % run) + rng.integers(-noise, noise + 1, n)).astype(np.int64)
+
+
 # GLC_SPLITMERGE_SEGMENT: partitions per speculative merge segment.  1 and 3
 # put group boundaries across segments everywhere, 2^30 is one segment (the
 # plain sequential walk); all must give the sequential walk's boundaries.
@@ -155,6 +167,11 @@ def test_gpu_splitmerge_boundaries(gpu, seg, monkeypatch):
              # values above 2^50 through the LDS split (m <= 128): integer residuals, prefix-sum refits
              (W.c3_int64(1 << 19, seed=7), 128, O.PART_SPLITMERGE_EXACT),
              (np.arange(300, dtype=np.uint32) * 7, 128, O.PART_CPU),
+             # the column's last start (fewer than m values) at each position of a four-start workgroup
+             (jumpy_runs(3001, 11, 97, 30), 7, O.PART_SPLITMERGE_EXACT),
+             (jumpy_runs(1000, 12, 41, 10), 3, O.PART_SPLITMERGE_EXACT),
+             (jumpy_runs(999, 13, 23, 5), 2, O.PART_SPLITMERGE_EXACT),
+             (jumpy_runs(4100, 14, 61, 20), 1, O.PART_SPLITMERGE_EXACT),
              (np.array([5], dtype=np.int64), 128, O.PART_CPU)]
     for v, m, mode in cases:
         print("split/merge case", v.dtype, v.size, m, "segment", seg, flush=True)
