@@ -30,6 +30,8 @@
 #include <algorithm>
 #include <cfloat>
 #include <cstdlib>
+#include <map>
+#include <mutex>
 #include <vector>
 
 #include "glc_device.hpp"
@@ -934,6 +936,25 @@ unsigned blocks(int64_t n, int b) { return static_cast<unsigned>((n + b - 1) / b
 // Partitions per speculative merge segment: GLC_SPLITMERGE_SEGMENT (1..2^30),
 // default 32.  Any value gives the same boundaries; it only moves work
 // between the parallel segment walks and the one-workgroup fix-up.
+// Scratch of one split/merge call, kept per device between calls and grown
+// when a larger column needs more (a per-call hipMalloc/hipFree pair costs
+// hundreds of microseconds and the free synchronises the device).  Held
+// until the process ends; the lock covers a whole call, which ends with its
+// stream synchronised.
+struct SmScratch {
+    std::mutex mu;
+    void* p = nullptr;
+    int64_t bytes = 0;
+};
+SmScratch& sm_scratch(int device) {
+    static std::mutex table_mu;
+    static std::map<int, SmScratch*>* table = new std::map<int, SmScratch*>();  // never freed: used until exit
+    std::lock_guard<std::mutex> g(table_mu);
+    SmScratch*& e = (*table)[device];
+    if (!e) e = new SmScratch();
+    return *e;
+}
+
 int64_t merge_segment_length() {
     const char* e = std::getenv("GLC_SPLITMERGE_SEGMENT");
     if (!e || !*e) return 32;
@@ -961,12 +982,24 @@ int gpu_split_merge(int dtype, const void* d_values, int64_t n, int32_t m, std::
     const int64_t q16 = staged ? align_up((J + 1) * 16, 256) : 0, c16 = staged ? align_up(C * 16, 256) : 0;
     const int64_t T = (J + kScanTile - 1) / kScanTile;  // scan tiles; tile totals and offsets: int32 x T each
     const int64_t t4 = align_up(T * 4, 256);
-    void* mem = nullptr;
-    if (hipMalloc(&mem, static_cast<size_t>(10 * i4 + 4 * d8 + 256 + 2 * q16 + 2 * c16 + 2 * t4)) != hipSuccess) {
-        (void)hipGetLastError();
-        return fail(GLC_ERR_OOM, "split/merge scratch allocation failed");
+    const int64_t bytes = 10 * i4 + 4 * d8 + 256 + 2 * q16 + 2 * c16 + 2 * t4;
+    int dev = 0;
+    GLC_HIP(hipGetDevice(&dev));
+    SmScratch& sc = sm_scratch(dev);
+    std::lock_guard<std::mutex> scratch_lock(sc.mu);
+    if (sc.bytes < bytes) {
+        if (sc.p) (void)hipFree(sc.p);  // the previous call ended synchronised
+        sc.p = nullptr;
+        sc.bytes = 0;
+        const int64_t want = align_up(bytes + bytes / 4, 1 << 20);
+        if (hipMalloc(&sc.p, static_cast<size_t>(want)) != hipSuccess) {
+            (void)hipGetLastError();
+            sc.p = nullptr;
+            return fail(GLC_ERR_OOM, "split/merge scratch allocation failed");
+        }
+        sc.bytes = want;
     }
-    char* p = static_cast<char*>(mem);
+    char* p = static_cast<char*>(sc.p);
     auto I = [&](int k) { return reinterpret_cast<int32_t*>(p + k * i4); };
     auto D = [&](int k) { return reinterpret_cast<double*>(p + 10 * i4 + k * d8); };
     int32_t *best_end = I(0), *jump0 = I(1), *jump1 = I(2), *mark = I(3), *pos = I(4);
@@ -1056,7 +1089,7 @@ int gpu_split_merge(int dtype, const void* d_values, int64_t n, int32_t m, std::
         GLC_HIP(hipStreamSynchronize(s));
         return GLC_OK;
     });
-    (void)hipFree(mem);
+    if (rc) (void)hipStreamSynchronize(s);  // no kernel of this call may outlive the lock on the scratch
     return rc;
 }
 

@@ -183,6 +183,38 @@ def test_gpu_splitmerge_boundaries(gpu, seg, monkeypatch):
             assert np.array_equal(ex.start, ref.start)
 
 
+def test_gpu_splitmerge_concurrent_calls(gpu):
+    """The split/merge scratch is kept per device and shared by every call:
+    calls from two threads at once (ctypes releases the GIL) on columns of
+    different sizes, so the scratch also grows while the other thread waits,
+    must give the boundaries of the same calls made one at a time."""
+    import threading
+    cols = [to_dev(W.c2_uint64(1 << 15, seed=31)), to_dev(linear_runs(1 << 18, 32, 20000, 4, np.int64)),
+            to_dev(W.c3_int64(1 << 16, seed=33)), to_dev(jumpy_runs(5000, 34, 70, 9))]
+    want = [glc.partition(x, glc.PART_GPU_SPLITMERGE, 128) for x in cols]
+    got = [[None] * len(cols) for _ in range(2)]
+    errors = []
+
+    def worker(t):
+        try:
+            order = range(len(cols)) if t == 0 else reversed(range(len(cols)))
+            for _ in range(3):
+                for k in order:
+                    got[t][k] = glc.partition(cols[k], glc.PART_GPU_SPLITMERGE, 128)
+        except Exception as e:  # reported below
+            errors.append(e)
+
+    ths = [threading.Thread(target=worker, args=(t,)) for t in range(2)]
+    for t in ths:
+        t.start()
+    for t in ths:
+        t.join()
+    assert not errors, errors
+    for t in range(2):
+        for k in range(len(cols)):
+            assert np.array_equal(got[t][k][0], want[k][0]) and np.array_equal(got[t][k][1], want[k][1]), (t, k)
+
+
 def test_parity_c1_config(gpu):
     # int32, N = 2^20, fixed 1024-value partitions, 10^5 uniform lookups
     v = W.c1_int32(1 << 20)
