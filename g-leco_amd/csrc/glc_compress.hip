@@ -205,7 +205,10 @@ __device__ __forceinline__ void fit_wave(const T* __restrict__ x, int32_t n, int
     fit_theta(n, Sx, Sy, Sxx, Sxy, t0, t1);
     int64_t mx = 0;
     const bool fast = enc_fast_range<T>(__builtin_fma(t1, 0.0, t0), __builtin_fma(t1, static_cast<double>(n - 1), t0));
-    for (int32_t base = 0; base < n; base += 2 * kFitWaveBs) {
+    // last step first: the end of the partition was read last by the sums
+    // pass and is the part most likely still in cache (the maximum does not
+    // depend on the order)
+    for (int32_t base = (n - 1) / (2 * kFitWaveBs) * (2 * kFitWaveBs); base >= 0; base -= 2 * kFitWaveBs) {
         T y[8];
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
