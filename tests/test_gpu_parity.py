@@ -164,6 +164,9 @@ def test_gpu_splitmerge_boundaries(gpu, seg, monkeypatch):
              (W.c1_int32(70000), 100, O.PART_CPU),
              (W.c2_uint64(1 << 17, seed=4), 128, O.PART_SPLITMERGE_EXACT),
              (W.c3_int64(1 << 17, seed=5), 256, O.PART_SPLITMERGE_EXACT),
+             # m > 128: sums by a pass over the values, FP64 residuals (values below 2^50)
+             (W.c2_uint64(1 << 17, seed=8), 256, O.PART_SPLITMERGE_EXACT),
+             (linear_runs(1 << 17, 9, 30000, 2, np.int64), 300, O.PART_SPLITMERGE_EXACT),
              # values above 2^50 through the LDS split (m <= 128): integer residuals, prefix-sum refits
              (W.c3_int64(1 << 19, seed=7), 128, O.PART_SPLITMERGE_EXACT),
              (np.arange(300, dtype=np.uint32) * 7, 128, O.PART_CPU),
