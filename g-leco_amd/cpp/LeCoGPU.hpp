@@ -17,6 +17,7 @@
 //     against the column (the reference trusts them, G:768-784).
 #pragma once
 
+#include <algorithm>
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
@@ -24,6 +25,7 @@
 #include <new>
 #include <string>
 #include <type_traits>
+#include <unordered_map>
 #include <vector>
 
 #include <hip/hip_runtime_api.h>
@@ -196,6 +198,46 @@ public:
     }
     void decompressFullFile_Fix(CompressedData<T>* cd, std::vector<T>& out, int partition_size) {
         full(cd, out, -partition_size);
+    }
+
+    // analyzeRandomAccessPerformance, G:116-164: host-only diagnostics of a
+    // query set printed to std::cout -- the mean gap between sorted positions,
+    // the share of gaps equal to 1, the number of fixed-size partitions the
+    // queries touch and queries per touched partition, and the reference's
+    // byte estimates (2*sizeof(T) per bit-packed lookup, 8 B per pre-unpacked
+    // lookup).  The column is not read.  Empty and one-element query sets and
+    // a non-positive partition size print 0 where the reference would divide
+    // by zero.
+    void analyzeRandomAccessPerformance(CompressedData<T>* compressed_data, const std::vector<int>& positions,
+                                        int fixed_partition_size) {
+        (void)compressed_data;
+        std::vector<int> sorted(positions);
+        std::sort(sorted.begin(), sorted.end());
+        long long gap_sum = 0, unit_gaps = 0;
+        for (size_t k = 1; k < sorted.size(); ++k) {
+            const long long g = static_cast<long long>(sorted[k]) - sorted[k - 1];
+            gap_sum += g;
+            unit_gaps += (g == 1);
+        }
+        const double gaps = sorted.size() > 1 ? static_cast<double>(sorted.size() - 1) : 0.0;
+        std::unordered_map<long long, long long> per_part;
+        if (fixed_partition_size > 0)
+            for (int p : positions) ++per_part[static_cast<long long>(p) / fixed_partition_size];
+        const double packed_bytes = static_cast<double>(positions.size()) * sizeof(T) * 2;
+        const double plain_bytes = static_cast<double>(positions.size()) * sizeof(long long);
+        std::cout << "\n--- Random Access Performance Analysis ---\n"
+                  << "Query pattern analysis:\n"
+                  << "  - Average stride: " << (gaps > 0 ? gap_sum / gaps : 0.0) << "\n"
+                  << "  - Sequential access ratio: " << (gaps > 0 ? unit_gaps / gaps : 0.0) << "\n"
+                  << "  - Total unique positions: " << positions.size() << "\n"
+                  << "  - Unique partitions accessed: " << per_part.size() << "\n"
+                  << "  - Queries per partition (avg): "
+                  << (per_part.empty() ? 0.0 : static_cast<double>(positions.size()) / per_part.size()) << "\n"
+                  << "\nMemory access analysis:\n"
+                  << "  - Bit-packed bytes accessed: ~" << static_cast<size_t>(packed_bytes) << "\n"
+                  << "  - Pre-unpacked bytes accessed: " << static_cast<size_t>(plain_bytes) << "\n"
+                  << "  - Memory access ratio: " << (packed_bytes > 0 ? plain_bytes / packed_bytes : 0.0) << "x"
+                  << std::endl;
     }
 
     // cleanup, G:793-821: frees the device arrays and the object.

@@ -10,6 +10,7 @@
 // usage: leco_app [n] [queries]
 #include <chrono>
 #include <cstdio>
+#include <cstring>
 #include <random>
 #include <string>
 #include <vector>
@@ -75,6 +76,7 @@ template <typename T> void run(const char* tname, size_t n, size_t nq) {
         leco.decompress(cd, pos, out);
         report("random access (decompress)", same(out, want));
         if (!c.var) {
+            leco.analyzeRandomAccessPerformance(cd, pos, TILE_SIZE);  // as app/Main.cu:151
             leco.randomAccessFixed(cd, pos, out, TILE_SIZE);
             report("randomAccessFixed", same(out, want));
             leco.decompressFullFile_Fix(cd, out, TILE_SIZE);
@@ -120,6 +122,40 @@ template <typename T> void run(const char* tname, size_t n, size_t nq) {
                     report("randomAccessFixedPreUnpacked", same(out, want));
                 }
                 leco.cleanup(d2);
+            }
+            {
+                // A version-4 blob (G:1029-1049): the v5 layout with version 4
+                // in the header, no 8-byte tail alignment, checksum resealed.
+                uint64_t delta_off = 0, delta_bytes = 0;
+                std::memcpy(&delta_off, s1->data + 72, 8);
+                std::memcpy(&delta_bytes, s1->data + 88, 8);
+                SerializedData v4;
+                v4.size = static_cast<size_t>(delta_off + delta_bytes);
+                v4.data = new uint8_t[v4.size];
+                std::memcpy(v4.data, s1->data, v4.size);
+                const uint32_t ver = 4, zero = 0;
+                std::memcpy(v4.data + 4, &ver, 4);
+                std::memcpy(v4.data + 100, &zero, 4);
+                const uint32_t csum = glc_header_checksum(v4.data, 120);
+                std::memcpy(v4.data + 100, &csum, 4);
+                CompressedData<T>* d4 = leco.deserializeGPU(&v4);
+                if (d4) leco.decompressFullFile(d4, out);
+                report("v4 blob: deserializeGPU -> decompressFullFile", d4 && same(out, data));
+                leco.cleanup(d4);
+                d4 = leco.deserialize(&v4);
+                if (d4) leco.decompress(d4, pos, out);
+                report("v4 blob: deserialize -> decompress", d4 && same(out, want));
+                leco.cleanup(d4);
+                DirectAccessHandle<T>* h4 = leco.createDirectAccessHandle(v4.data, v4.size, true);
+                bool ok4 = h4 != nullptr;
+                if (h4) {
+                    leco.directRandomAccessCPU(h4, pos, out);
+                    ok4 = ok4 && same(out, want);
+                    leco.directRandomAccessGPU(h4, pos, out);
+                    ok4 = ok4 && same(out, want);
+                    leco.destroyDirectAccessHandle(h4);
+                }
+                report("v4 blob: direct access (CPU, GPU)", ok4);
             }
             const std::string path = std::string("/tmp/leco_app_") + tname + ".bin";
             bool saved = leco.saveToFile(s1->data, s1->size, path);

@@ -175,26 +175,27 @@ unsigned blocks_for(int64_t work, int threads) {
 }
 
 int compress_impl(int dtype, const void* d_values, int64_t n, int partitioner, int32_t S, int32_t base,
-                  int32_t min_size, const int32_t* h_start, const int32_t* h_end, int64_t P_explicit, hipStream_t s,
-                  glc_column** out) {
+                  int32_t min_size, int32_t fit_form, const int32_t* h_start, const int32_t* h_end, int64_t P_explicit,
+                  hipStream_t s, glc_column** out) {
     if (!out) return fail(GLC_ERR_INVALID_ARG, "out is NULL");
     *out = nullptr;
     if (!valid_dtype(dtype)) return fail(GLC_ERR_INVALID_ARG, "bad dtype");
     if (n < 0 || n > 0x7FFFFFFFLL) return fail(GLC_ERR_INVALID_ARG, "n must be in [0, 2^31) (int32 indices, H21)");
     if (n > 0 && !d_values) return fail(GLC_ERR_INVALID_ARG, "values is NULL");
+    if (fit_form != 0 && fit_form != 1) return fail(GLC_ERR_INVALID_ARG, "fit_form must be 0 or 1");
     glc_column* c = new (std::nothrow) glc_column();
     if (!c) return fail(GLC_ERR_OOM, "host allocation failed");
     c->dtype = dtype;
     c->elem_size = elem_size_of(dtype);
-    GLC_HIP(hipGetDevice(&c->device));
+    auto bail = [&](int code) { destroy(c); return code; };
+    GLC_HIP_BAIL(hipGetDevice(&c->device));
     c->n = n;
     int rc = GLC_OK;
     int64_t len0 = 0;
-    auto bail = [&](int code) { destroy(c); return code; };
     if (n == 0) {  // G:355-371: an empty column
         if ((rc = alloc_column(c, 0, s))) return bail(rc);
         if ((rc = alloc_payload(c, 0, s))) return bail(rc);
-        GLC_HIP(hipStreamSynchronize(s));
+        GLC_HIP_BAIL(hipStreamSynchronize(s));
         *out = c;
         return GLC_OK;
     }
@@ -205,7 +206,7 @@ int compress_impl(int dtype, const void* d_values, int64_t n, int partitioner, i
             if ((rc = alloc_column(c, P, s))) return bail(rc);
             hipLaunchKernelGGL(k_fixed_parts, dim3(blocks_for(P, 256)), dim3(256), 0, s, c->d_start, c->d_end, P, n,
                                S);
-            GLC_HIP(hipGetLastError());
+            GLC_HIP_BAIL(hipGetLastError());
             len0 = std::min<int64_t>(S, n);
             break;
         }
@@ -219,7 +220,7 @@ int compress_impl(int dtype, const void* d_values, int64_t n, int partitioner, i
             std::lock_guard<std::mutex> scratch_lock(sc.mu);
             if (sc.bytes < bytes) {
                 if (sc.p) {
-                    GLC_HIP(hipStreamSynchronize(s));
+                    GLC_HIP_BAIL(hipStreamSynchronize(s));
                     (void)hipFree(sc.p);
                 }
                 sc.p = nullptr;
@@ -258,22 +259,22 @@ int compress_impl(int dtype, const void* d_values, int64_t n, int partitioner, i
             std::vector<int32_t> st, en;
             if ((rc = gpu_split_merge(dtype, d_values, n, min_size, st, en, s))) return bail(rc);
             if ((rc = alloc_column(c, static_cast<int64_t>(st.size()), s))) return bail(rc);
-            GLC_HIP(hipMemcpyAsync(c->d_start, st.data(), st.size() * 4, hipMemcpyHostToDevice, s));
-            GLC_HIP(hipMemcpyAsync(c->d_end, en.data(), en.size() * 4, hipMemcpyHostToDevice, s));
-            GLC_HIP(hipStreamSynchronize(s));
+            GLC_HIP_BAIL(hipMemcpyAsync(c->d_start, st.data(), st.size() * 4, hipMemcpyHostToDevice, s));
+            GLC_HIP_BAIL(hipMemcpyAsync(c->d_end, en.data(), en.size() * 4, hipMemcpyHostToDevice, s));
+            GLC_HIP_BAIL(hipStreamSynchronize(s));
             len0 = en[0] - st[0];
             break;
         }
         case GLC_PART_CPU_SPLITMERGE: {
             std::vector<char> h(static_cast<size_t>(n) * c->elem_size);
-            GLC_HIP(hipMemcpyAsync(h.data(), d_values, h.size(), hipMemcpyDeviceToHost, s));
-            GLC_HIP(hipStreamSynchronize(s));
+            GLC_HIP_BAIL(hipMemcpyAsync(h.data(), d_values, h.size(), hipMemcpyDeviceToHost, s));
+            GLC_HIP_BAIL(hipStreamSynchronize(s));
             std::vector<int32_t> st, en;
             if ((rc = cpu_split_merge(dtype, h.data(), n, min_size, st, en))) return bail(rc);
             if ((rc = alloc_column(c, static_cast<int64_t>(st.size()), s))) return bail(rc);
-            GLC_HIP(hipMemcpyAsync(c->d_start, st.data(), st.size() * 4, hipMemcpyHostToDevice, s));
-            GLC_HIP(hipMemcpyAsync(c->d_end, en.data(), en.size() * 4, hipMemcpyHostToDevice, s));
-            GLC_HIP(hipStreamSynchronize(s));
+            GLC_HIP_BAIL(hipMemcpyAsync(c->d_start, st.data(), st.size() * 4, hipMemcpyHostToDevice, s));
+            GLC_HIP_BAIL(hipMemcpyAsync(c->d_end, en.data(), en.size() * 4, hipMemcpyHostToDevice, s));
+            GLC_HIP_BAIL(hipStreamSynchronize(s));
             len0 = en[0] - st[0];
             break;
         }
@@ -287,18 +288,19 @@ int compress_impl(int dtype, const void* d_values, int64_t n, int partitioner, i
                     return bail(fail(GLC_ERR_FORMAT, "partitions must be sorted, non-overlapping and inside [0, N)"));
             }
             if ((rc = alloc_column(c, P_explicit, s))) return bail(rc);
-            GLC_HIP(hipMemcpyAsync(c->d_start, h_start, P_explicit * 4, hipMemcpyHostToDevice, s));
-            GLC_HIP(hipMemcpyAsync(c->d_end, h_end, P_explicit * 4, hipMemcpyHostToDevice, s));
-            GLC_HIP(hipStreamSynchronize(s));
+            GLC_HIP_BAIL(hipMemcpyAsync(c->d_start, h_start, P_explicit * 4, hipMemcpyHostToDevice, s));
+            GLC_HIP_BAIL(hipMemcpyAsync(c->d_end, h_end, P_explicit * 4, hipMemcpyHostToDevice, s));
+            GLC_HIP_BAIL(hipStreamSynchronize(s));
             len0 = static_cast<int64_t>(h_end[0]) - h_start[0];
             break;
         }
         default: return bail(fail(GLC_ERR_INVALID_ARG, "bad partitioner"));
     }
     hipLaunchKernelGGL(k_init_types, dim3(blocks_for(c->P, 256)), dim3(256), 0, s, c->d_type, c->P);
-    GLC_HIP(hipGetLastError());
+    GLC_HIP_BAIL(hipGetLastError());
     c->fit_block = fit_block_size(len0);
-    if ((rc = launch_fit(dtype, d_values, c, c->fit_block, s))) return bail(rc);
+    c->fit_form = fit_form;
+    if ((rc = launch_fit(dtype, d_values, c, c->fit_block, fit_form, s))) return bail(rc);
     int64_t* d_total = reinterpret_cast<int64_t*>(summary_ptr(c)) + 8;  // scratch slot after the summary
     if ((rc = launch_scan_offsets(c, d_total, s))) return bail(rc);
     IndexSummary sum;
@@ -307,7 +309,7 @@ int compress_impl(int dtype, const void* d_values, int64_t n, int partitioner, i
     c->fixed_size = sum.not_fixed ? 0 : static_cast<int32_t>(len0);
     if ((rc = alloc_payload(c, (c->total_bits + 31) / 32, s))) return bail(rc);
     if ((rc = launch_pack(dtype, d_values, c, s))) return bail(rc);
-    GLC_HIP(hipStreamSynchronize(s));
+    GLC_HIP_BAIL(hipStreamSynchronize(s));
     *out = c;
     return GLC_OK;
 }
@@ -430,12 +432,12 @@ int glc_compress(int dtype, const void* d_values, int64_t n, const glc_opts* opt
     if (o.partitioner == GLC_PART_EXPLICIT)
         return fail(GLC_ERR_INVALID_ARG, "use glc_compress_partitions for explicit partitions");
     return compress_impl(dtype, d_values, n, o.partitioner, o.partition_size, o.v6_base_size, o.partition_size,
-                         nullptr, nullptr, 0, static_cast<hipStream_t>(stream), out);
+                         o.fit_form, nullptr, nullptr, 0, static_cast<hipStream_t>(stream), out);
 }
 
 int glc_compress_partitions(int dtype, const void* d_values, int64_t n, const int32_t* h_start, const int32_t* h_end,
                             int64_t num_partitions, void* stream, glc_column** out) {
-    return compress_impl(dtype, d_values, n, GLC_PART_EXPLICIT, 0, 0, 0, h_start, h_end, num_partitions,
+    return compress_impl(dtype, d_values, n, GLC_PART_EXPLICIT, 0, 0, 0, 0, h_start, h_end, num_partitions,
                          static_cast<hipStream_t>(stream), out);
 }
 
@@ -643,16 +645,20 @@ static int serialize_common(const glc_column* c, hipStream_t s, bool device_asse
         }
     } else {
         // serialize, G:826-1016: per-array D2H copies into the host blob.
+        auto bail = [&](int code) {
+            std::free(b);
+            return code;
+        };
         if (P) {
-            GLC_HIP(hipMemcpy(b + h.start_off, c->d_start, P * 4, hipMemcpyDeviceToHost));
-            GLC_HIP(hipMemcpy(b + h.end_off, c->d_end, P * 4, hipMemcpyDeviceToHost));
-            GLC_HIP(hipMemcpy(b + h.types_off, c->d_type, P * 4, hipMemcpyDeviceToHost));
-            GLC_HIP(hipMemcpy(b + h.params_off, c->d_params, P * 32, hipMemcpyDeviceToHost));
-            GLC_HIP(hipMemcpy(b + h.bits_off, c->d_bits, P * 4, hipMemcpyDeviceToHost));
-            GLC_HIP(hipMemcpy(b + h.bitoffs_off, c->d_off, P * 8, hipMemcpyDeviceToHost));
-            GLC_HIP(hipMemcpy(b + h.err_off, c->d_err, P * 8, hipMemcpyDeviceToHost));
+            GLC_HIP_BAIL(hipMemcpy(b + h.start_off, c->d_start, P * 4, hipMemcpyDeviceToHost));
+            GLC_HIP_BAIL(hipMemcpy(b + h.end_off, c->d_end, P * 4, hipMemcpyDeviceToHost));
+            GLC_HIP_BAIL(hipMemcpy(b + h.types_off, c->d_type, P * 4, hipMemcpyDeviceToHost));
+            GLC_HIP_BAIL(hipMemcpy(b + h.params_off, c->d_params, P * 32, hipMemcpyDeviceToHost));
+            GLC_HIP_BAIL(hipMemcpy(b + h.bits_off, c->d_bits, P * 4, hipMemcpyDeviceToHost));
+            GLC_HIP_BAIL(hipMemcpy(b + h.bitoffs_off, c->d_off, P * 8, hipMemcpyDeviceToHost));
+            GLC_HIP_BAIL(hipMemcpy(b + h.err_off, c->d_err, P * 8, hipMemcpyDeviceToHost));
         }
-        if (h.delta_bytes) GLC_HIP(hipMemcpy(b + h.delta_off, c->d_words, h.delta_bytes, hipMemcpyDeviceToHost));
+        if (h.delta_bytes) GLC_HIP_BAIL(hipMemcpy(b + h.delta_off, c->d_words, h.delta_bytes, hipMemcpyDeviceToHost));
     }
     *blob = b;
     *size = total;
@@ -679,22 +685,22 @@ int glc_deserialize(const uint8_t* blob, size_t size, int dtype, int preunpack, 
     if (!c) return fail(GLC_ERR_OOM, "host allocation failed");
     c->dtype = dtype;
     c->elem_size = elem_size_of(dtype);
-    GLC_HIP(hipGetDevice(&c->device));
+    auto bail = [&](int code) { destroy(c); return code; };
+    GLC_HIP_BAIL(hipGetDevice(&c->device));
     c->n = h.total_values;
     const int64_t P = h.num_partitions;
-    auto bail = [&](int code) { destroy(c); return code; };
     if ((rc = alloc_column(c, P, s))) return bail(rc);
     if (P) {
-        GLC_HIP(hipMemcpyAsync(c->d_start, blob + h.start_off, P * 4, hipMemcpyHostToDevice, s));
-        GLC_HIP(hipMemcpyAsync(c->d_end, blob + h.end_off, P * 4, hipMemcpyHostToDevice, s));
-        GLC_HIP(hipMemcpyAsync(c->d_type, blob + h.types_off, P * 4, hipMemcpyHostToDevice, s));
-        GLC_HIP(hipMemcpyAsync(c->d_params, blob + h.params_off, P * 32, hipMemcpyHostToDevice, s));
-        GLC_HIP(hipMemcpyAsync(c->d_bits, blob + h.bits_off, P * 4, hipMemcpyHostToDevice, s));
-        GLC_HIP(hipMemcpyAsync(c->d_off, blob + h.bitoffs_off, P * 8, hipMemcpyHostToDevice, s));
-        GLC_HIP(hipMemcpyAsync(c->d_err, blob + h.err_off, P * 8, hipMemcpyHostToDevice, s));
+        GLC_HIP_BAIL(hipMemcpyAsync(c->d_start, blob + h.start_off, P * 4, hipMemcpyHostToDevice, s));
+        GLC_HIP_BAIL(hipMemcpyAsync(c->d_end, blob + h.end_off, P * 4, hipMemcpyHostToDevice, s));
+        GLC_HIP_BAIL(hipMemcpyAsync(c->d_type, blob + h.types_off, P * 4, hipMemcpyHostToDevice, s));
+        GLC_HIP_BAIL(hipMemcpyAsync(c->d_params, blob + h.params_off, P * 32, hipMemcpyHostToDevice, s));
+        GLC_HIP_BAIL(hipMemcpyAsync(c->d_bits, blob + h.bits_off, P * 4, hipMemcpyHostToDevice, s));
+        GLC_HIP_BAIL(hipMemcpyAsync(c->d_off, blob + h.bitoffs_off, P * 8, hipMemcpyHostToDevice, s));
+        GLC_HIP_BAIL(hipMemcpyAsync(c->d_err, blob + h.err_off, P * 8, hipMemcpyHostToDevice, s));
     }
     if ((rc = alloc_payload(c, static_cast<int64_t>((h.delta_bytes + 3) / 4), s))) return bail(rc);
-    if (h.delta_bytes) GLC_HIP(hipMemcpyAsync(c->d_words, blob + h.delta_off, h.delta_bytes, hipMemcpyHostToDevice, s));
+    if (h.delta_bytes) GLC_HIP_BAIL(hipMemcpyAsync(c->d_words, blob + h.delta_off, h.delta_bytes, hipMemcpyHostToDevice, s));
     IndexSummary sum;
     if ((rc = finish_index(c, s, &sum))) return bail(rc);
     if (static_cast<uint64_t>(sum.total_bits) > h.delta_bytes * 8)
@@ -703,14 +709,14 @@ int glc_deserialize(const uint8_t* blob, size_t size, int dtype, int preunpack, 
     c->fixed_size = 0;
     if (P > 0 && !sum.not_fixed) {
         int32_t s0 = 0, e0 = 0;
-        GLC_HIP(hipMemcpy(&s0, c->d_start, 4, hipMemcpyDeviceToHost));
-        GLC_HIP(hipMemcpy(&e0, c->d_end, 4, hipMemcpyDeviceToHost));
+        GLC_HIP_BAIL(hipMemcpy(&s0, c->d_start, 4, hipMemcpyDeviceToHost));
+        GLC_HIP_BAIL(hipMemcpy(&e0, c->d_end, 4, hipMemcpyDeviceToHost));
         c->fixed_size = e0 - s0;
     }
     if (preunpack && c->n > 0 && c->P > 0 && c->payload_words > 0) {  // G:2249-2288
         if ((rc = glc_preunpack(c, 1, s))) return bail(rc);
     }
-    GLC_HIP(hipStreamSynchronize(s));
+    GLC_HIP_BAIL(hipStreamSynchronize(s));
     *out = c;
     return GLC_OK;
 }

@@ -45,8 +45,10 @@ struct glc_batch {
     std::vector<void*> cur_outs;
     hipEvent_t ev_outs = nullptr;
     bool outs_pending = false;
-    hipEvent_t ev_comp = nullptr;
+    hipEvent_t ev_comp = nullptr;  // after the last compress enqueued (any stream)
     bool compress_enqueued = false;
+    hipEvent_t ev_dec = nullptr;   // after the last decompress enqueued (any stream)
+    bool decode_enqueued = false;
     bool sizes_valid = false;
     std::vector<int64_t> bits;
     std::vector<glc_column*> views;
@@ -72,6 +74,7 @@ void release(glc_batch* b) {
     if (b->cap) (void)hipStreamDestroy(b->cap);
     if (b->ev_outs) (void)hipEventDestroy(b->ev_outs);
     if (b->ev_comp) (void)hipEventDestroy(b->ev_comp);
+    if (b->ev_dec) (void)hipEventDestroy(b->ev_dec);
     if (b->h_outs) (void)hipHostFree(b->h_outs);
     if (b->meta) (void)hipFree(b->meta);
     if (b->words) (void)hipFree(b->words);
@@ -118,10 +121,27 @@ int enqueue(glc_batch* b, int what, hipStream_t s) {
     return GLC_OK;
 }
 
+// Every call orders its device work after the batch's previous compress and
+// decompress, whatever streams those were enqueued on: a compress rewrites the
+// metadata and payload a decode reads, and a decode reads the shared output
+// table that the next decode's upload_outs rewrites.  On one stream the waits
+// are already satisfied by stream order.
+int order_after_previous(glc_batch* b, hipStream_t s) {
+    if (b->compress_enqueued) GLC_HIP(hipStreamWaitEvent(s, b->ev_comp, 0));
+    if (b->decode_enqueued) GLC_HIP(hipStreamWaitEvent(s, b->ev_dec, 0));
+    return GLC_OK;
+}
+
 int note_compress(glc_batch* b, hipStream_t s) {
     GLC_HIP(hipEventRecord(b->ev_comp, s));
     b->compress_enqueued = true;
     b->sizes_valid = false;
+    return GLC_OK;
+}
+
+int note_decode(glc_batch* b, hipStream_t s) {
+    GLC_HIP(hipEventRecord(b->ev_dec, s));
+    b->decode_enqueued = true;
     return GLC_OK;
 }
 
@@ -270,6 +290,7 @@ int glc_batch_create(int num_columns, const int32_t* dtypes, const void* const* 
         return bail(fail(GLC_ERR_OOM, "pinned allocation of the output table failed"));
     if (hipEventCreateWithFlags(&b->ev_outs, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&b->ev_comp, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b->ev_dec, hipEventDisableTiming) != hipSuccess ||
         hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(GLC_ERR_HIP, "event/stream creation failed"));
     b->views.assign(num_columns, nullptr);
@@ -281,8 +302,9 @@ int glc_batch_compress(glc_batch* b, void* stream) {
     if (!b) return fail(GLC_ERR_INVALID_ARG, "batch is null");
     GLC_HIP(hipSetDevice(b->device));
     hipStream_t s = static_cast<hipStream_t>(stream);
-    int rc = enqueue(b, GLC_BATCH_COMPRESS, s);
+    int rc = order_after_previous(b, s);
     if (rc) return rc;
+    if ((rc = enqueue(b, GLC_BATCH_COMPRESS, s))) return rc;
     return note_compress(b, s);
 }
 
@@ -292,9 +314,11 @@ int glc_batch_decompress(glc_batch* b, void* const* d_out, void* stream) {
     if (!b->compress_enqueued) return fail(GLC_ERR_STATE, "batch has not been compressed");
     GLC_HIP(hipSetDevice(b->device));
     hipStream_t s = static_cast<hipStream_t>(stream);
-    int rc = upload_outs(b, d_out, s);
+    int rc = order_after_previous(b, s);
     if (rc) return rc;
-    return enqueue(b, GLC_BATCH_DECOMPRESS, s);
+    if ((rc = upload_outs(b, d_out, s))) return rc;
+    if ((rc = enqueue(b, GLC_BATCH_DECOMPRESS, s))) return rc;
+    return note_decode(b, s);
 }
 
 int glc_batch_run_graph(glc_batch* b, int what, void* const* d_out, void* stream) {
@@ -304,8 +328,12 @@ int glc_batch_run_graph(glc_batch* b, int what, void* const* d_out, void* stream
         return fail(GLC_ERR_STATE, "batch has not been compressed");
     GLC_HIP(hipSetDevice(b->device));
     hipStream_t s = static_cast<hipStream_t>(stream);
+    if ((what & GLC_BATCH_DECOMPRESS) && !d_out && b->C > 0) return fail(GLC_ERR_INVALID_ARG, "d_out is null");
+    {
+        int rc = order_after_previous(b, s);
+        if (rc) return rc;
+    }
     if (what & GLC_BATCH_DECOMPRESS) {
-        if (!d_out && b->C > 0) return fail(GLC_ERR_INVALID_ARG, "d_out is null");
         int rc = upload_outs(b, d_out, s);
         if (rc) return rc;
     }
@@ -336,7 +364,11 @@ int glc_batch_run_graph(glc_batch* b, int what, void* const* d_out, void* stream
         b->graph_sched[what][1] = b->tpw;
     }
     GLC_HIP(hipGraphLaunch(g, s));
-    if (what & GLC_BATCH_COMPRESS) return note_compress(b, s);
+    if (what & GLC_BATCH_COMPRESS) {
+        int rc = note_compress(b, s);
+        if (rc) return rc;
+    }
+    if (what & GLC_BATCH_DECOMPRESS) return note_decode(b, s);
     return GLC_OK;
 }
 
@@ -365,8 +397,9 @@ int glc_batch_autotune(glc_batch* b, void* const* d_out, int32_t reps, void* str
     if (!b->compress_enqueued) return fail(GLC_ERR_STATE, "batch has not been compressed");
     GLC_HIP(hipSetDevice(b->device));
     hipStream_t s = static_cast<hipStream_t>(stream);
-    int rc = upload_outs(b, d_out, s);
+    int rc = order_after_previous(b, s);
     if (rc) return rc;
+    if ((rc = upload_outs(b, d_out, s))) return rc;
     hipEvent_t e0, e1;
     GLC_HIP(hipEventCreate(&e0));
     if (hipEventCreate(&e1) != hipSuccess) {
@@ -396,7 +429,9 @@ int glc_batch_autotune(glc_batch* b, void* const* d_out, int32_t reps, void* str
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    const int rn = note_decode(b, s);  // decodes were enqueued even when a later step failed
     if (rc) return rc;
+    if (rn) return rn;
     b->wpg = bw;
     b->tpw = bt;
     if (waves_per_group) *waves_per_group = bw;

@@ -36,15 +36,28 @@ struct FitSmem {
     int64_t mx[kFitThreads / 64];
 };
 
+// Contraction form of the fit's FP64 expressions (:113-158 write a*b + c; the
+// reference build lets nvcc fuse them, which the source does not show):
+//   FORM 0: sum_xx += x*x and sum_xy += x*y as fma(x, x, s) / fma(x, y, s),
+//           det = fma(n, Sxx, -(Sx*Sx)), num = fma(n, Sxy, -(Sx*Sy)),
+//           theta0 = fma(-theta1, Sx, Sy) / n            (the default)
+//   FORM 1: every product rounded on its own, no fma.
+// The prediction fma(theta1, i, theta0) is the same in both forms.
+template <int FORM> __device__ __forceinline__ double acc_prod(double a, double b, double s) {
+    if constexpr (FORM == 0) return __builtin_fma(a, b, s);
+    else return s + a * b;
+}
+
 // theta from the four sums (:140-160): determinant test, then slope and
-// intercept, each with one fma.
+// intercept.
+template <int FORM>
 __device__ __forceinline__ void fit_theta(int32_t n, double Sx, double Sy, double Sxx, double Sxy, double& t0,
                                           double& t1) {
     const double dn = static_cast<double>(n);
-    const double det = __builtin_fma(dn, Sxx, -(Sx * Sx));
+    const double det = (FORM == 0) ? __builtin_fma(dn, Sxx, -(Sx * Sx)) : dn * Sxx - Sx * Sx;
     if (__builtin_fabs(det) > 1e-10) {
-        t1 = __builtin_fma(dn, Sxy, -(Sx * Sy)) / det;
-        t0 = __builtin_fma(-t1, Sx, Sy) / dn;
+        t1 = ((FORM == 0) ? __builtin_fma(dn, Sxy, -(Sx * Sy)) : dn * Sxy - Sx * Sy) / det;
+        t0 = ((FORM == 0) ? __builtin_fma(-t1, Sx, Sy) : Sy - t1 * Sx) / dn;
     } else {
         t1 = 0.0;
         t0 = Sy / dn;
@@ -64,7 +77,7 @@ __device__ __forceinline__ int32_t bits_for(int64_t m) {
 }
 
 // Partition p of n values at x (n > 0).  Called by the whole workgroup.
-template <class T>
+template <class T, int FORM>
 __device__ __forceinline__ void fit_one(const T* __restrict__ x, int32_t n, int bs, int64_t p, int32_t* __restrict__ type,
                                         double* __restrict__ params, int32_t* __restrict__ bits,
                                         int64_t* __restrict__ err, FitSmem& sm) {
@@ -79,8 +92,8 @@ __device__ __forceinline__ void fit_one(const T* __restrict__ x, int32_t n, int 
             const double yi = static_cast<double>(val);
             a = a + xi;
             b = b + yi;
-            c = __builtin_fma(xi, xi, c);
-            d = __builtin_fma(xi, yi, d);
+            c = acc_prod<FORM>(xi, xi, c);
+            d = acc_prod<FORM>(xi, yi, d);
         }
     }
     if (__syncthreads_or(ovf)) {  // :95-106 DIRECT_COPY
@@ -106,7 +119,7 @@ __device__ __forceinline__ void fit_one(const T* __restrict__ x, int32_t n, int 
     }
     if (tid == 0) {
         double t0, t1;
-        fit_theta(n, sm.sum[0][0], sm.sum[1][0], sm.sum[2][0], sm.sum[3][0], t0, t1);
+        fit_theta<FORM>(n, sm.sum[0][0], sm.sum[1][0], sm.sum[2][0], sm.sum[3][0], t0, t1);
         sm.theta[0] = t0; sm.theta[1] = t1;
         type[p] = MODEL_LINEAR;
         params[4 * p] = t0; params[4 * p + 1] = t1; params[4 * p + 2] = 0.0; params[4 * p + 3] = 0.0;
@@ -145,7 +158,7 @@ constexpr int kFitWaves = 4;
 constexpr int kFitWaveBs = 256;
 
 // Partition p of n > 0 values at x, fitted by one wave.
-template <class T>
+template <class T, int FORM>
 __device__ __forceinline__ void fit_wave(const T* __restrict__ x, int32_t n, int64_t p, int32_t* __restrict__ type,
                                          double* __restrict__ params, int32_t* __restrict__ bits,
                                          int64_t* __restrict__ err, int lane) {
@@ -171,8 +184,8 @@ __device__ __forceinline__ void fit_wave(const T* __restrict__ x, int32_t n, int
                 const double yi = static_cast<double>(y[k]);
                 a[k & 3] = a[k & 3] + xi;
                 b[k & 3] = b[k & 3] + yi;
-                c[k & 3] = __builtin_fma(xi, xi, c[k & 3]);
-                d[k & 3] = __builtin_fma(xi, yi, d[k & 3]);
+                c[k & 3] = acc_prod<FORM>(xi, xi, c[k & 3]);
+                d[k & 3] = acc_prod<FORM>(xi, yi, d[k & 3]);
             }
         }
     }
@@ -202,7 +215,7 @@ __device__ __forceinline__ void fit_wave(const T* __restrict__ x, int32_t n, int
     Sxx = __shfl(Sxx, 0, 64);
     Sxy = __shfl(Sxy, 0, 64);
     double t0, t1;
-    fit_theta(n, Sx, Sy, Sxx, Sxy, t0, t1);
+    fit_theta<FORM>(n, Sx, Sy, Sxx, Sxy, t0, t1);
     int64_t mx = 0;
     const bool fast = enc_fast_range<T>(__builtin_fma(t1, 0.0, t0), __builtin_fma(t1, static_cast<double>(n - 1), t0));
     // last step first: the end of the partition was read last by the sums
@@ -245,7 +258,7 @@ __device__ __forceinline__ void fit_wave(const T* __restrict__ x, int32_t n, int
     }
 }
 
-template <class T>
+template <class T, int FORM>
 __global__ __launch_bounds__(64 * kFitWaves) void k_fit_wave(const T* __restrict__ v, const int32_t* __restrict__ st,
                                                             const int32_t* __restrict__ en, int32_t* __restrict__ type,
                                                             double* __restrict__ params, int32_t* __restrict__ bits,
@@ -256,10 +269,10 @@ __global__ __launch_bounds__(64 * kFitWaves) void k_fit_wave(const T* __restrict
     const int32_t s0 = st[p];
     const int32_t n = en[p] - s0;
     if (n <= 0) return;  // :62, entry keeps its initial values
-    fit_wave<T>(v + s0, n, p, type, params, bits, err, lane);
+    fit_wave<T, FORM>(v + s0, n, p, type, params, bits, err, lane);
 }
 
-template <class T>
+template <class T, int FORM>
 __global__ __launch_bounds__(kFitThreads) void k_fit(const T* __restrict__ v, const int32_t* __restrict__ st,
                                                      const int32_t* __restrict__ en, int32_t* __restrict__ type,
                                                      double* __restrict__ params, int32_t* __restrict__ bits,
@@ -269,7 +282,7 @@ __global__ __launch_bounds__(kFitThreads) void k_fit(const T* __restrict__ v, co
         const int32_t s0 = st[p];
         const int32_t n = en[p] - s0;
         if (n <= 0) continue;  // :62, entry keeps its initial values
-        fit_one<T>(v + s0, n, bs, p, type, params, bits, err, sm);
+        fit_one<T, FORM>(v + s0, n, bs, p, type, params, bits, err, sm);
     }
 }
 
@@ -287,7 +300,7 @@ __global__ __launch_bounds__(kFitThreads) void k_fit_batch(const BatchCol* __res
         const int32_t s0 = c.start[p];
         const int32_t n = c.end[p] - s0;
         if (n <= 0) continue;
-        fit_one<T>(static_cast<const T*>(c.values) + s0, n, c.fit_bs, p, c.type, c.params, c.bits, c.err, sm);
+        fit_one<T, 0>(static_cast<const T*>(c.values) + s0, n, c.fit_bs, p, c.type, c.params, c.bits, c.err, sm);
     }
 }
 
@@ -305,10 +318,10 @@ __global__ __launch_bounds__(64 * kFitWaves) void k_fit_wave_batch(const BatchCo
     const int32_t n = c.end[p] - s0;
     if (n <= 0) return;
     switch (c.dtype) {
-        case GLC_I32: fit_wave<int32_t>(static_cast<const int32_t*>(c.values) + s0, n, p, c.type, c.params, c.bits, c.err, lane); break;
-        case GLC_U32: fit_wave<uint32_t>(static_cast<const uint32_t*>(c.values) + s0, n, p, c.type, c.params, c.bits, c.err, lane); break;
-        case GLC_I64: fit_wave<int64_t>(static_cast<const int64_t*>(c.values) + s0, n, p, c.type, c.params, c.bits, c.err, lane); break;
-        default: fit_wave<uint64_t>(static_cast<const uint64_t*>(c.values) + s0, n, p, c.type, c.params, c.bits, c.err, lane); break;
+        case GLC_I32: fit_wave<int32_t, 0>(static_cast<const int32_t*>(c.values) + s0, n, p, c.type, c.params, c.bits, c.err, lane); break;
+        case GLC_U32: fit_wave<uint32_t, 0>(static_cast<const uint32_t*>(c.values) + s0, n, p, c.type, c.params, c.bits, c.err, lane); break;
+        case GLC_I64: fit_wave<int64_t, 0>(static_cast<const int64_t*>(c.values) + s0, n, p, c.type, c.params, c.bits, c.err, lane); break;
+        default: fit_wave<uint64_t, 0>(static_cast<const uint64_t*>(c.values) + s0, n, p, c.type, c.params, c.bits, c.err, lane); break;
     }
 }
 
@@ -1141,23 +1154,28 @@ inline unsigned grid_for(int64_t work, int64_t per_block, int64_t cap = 1LL << 3
 
 }  // namespace
 
-int launch_fit(int dtype, const void* d_values, glc_column* c, int32_t bs, hipStream_t s) {
-    if (c->P == 0) return GLC_OK;
-    if (bs < 32 || bs > kFitThreads) return fail(GLC_ERR_INVALID_ARG, "fit block size out of range");
-    return dispatch_dtype(dtype, [&](auto t) {
-        using T = decltype(t);
-        if (bs == kFitWaveBs) {
-            const unsigned g = static_cast<unsigned>((c->P + kFitWaves - 1) / kFitWaves);
-            hipLaunchKernelGGL(k_fit_wave<T>, dim3(g), dim3(64 * kFitWaves), 0, s, static_cast<const T*>(d_values),
-                               c->d_start, c->d_end, c->d_type, c->d_params, c->d_bits, c->d_err, c->P);
-            GLC_HIP(hipGetLastError());
-            return GLC_OK;
-        }
-        const unsigned g = static_cast<unsigned>(c->P < (1LL << 30) ? c->P : (1LL << 30));
-        hipLaunchKernelGGL(k_fit<T>, dim3(g), dim3(kFitThreads), 0, s, static_cast<const T*>(d_values), c->d_start,
-                           c->d_end, c->d_type, c->d_params, c->d_bits, c->d_err, c->P, bs);
+template <class T, int FORM> int launch_fit_t(const void* d_values, glc_column* c, int32_t bs, hipStream_t s) {
+    if (bs == kFitWaveBs) {
+        const unsigned g = static_cast<unsigned>((c->P + kFitWaves - 1) / kFitWaves);
+        hipLaunchKernelGGL((k_fit_wave<T, FORM>), dim3(g), dim3(64 * kFitWaves), 0, s, static_cast<const T*>(d_values),
+                           c->d_start, c->d_end, c->d_type, c->d_params, c->d_bits, c->d_err, c->P);
         GLC_HIP(hipGetLastError());
         return GLC_OK;
+    }
+    const unsigned g = static_cast<unsigned>(c->P < (1LL << 30) ? c->P : (1LL << 30));
+    hipLaunchKernelGGL((k_fit<T, FORM>), dim3(g), dim3(kFitThreads), 0, s, static_cast<const T*>(d_values), c->d_start,
+                       c->d_end, c->d_type, c->d_params, c->d_bits, c->d_err, c->P, bs);
+    GLC_HIP(hipGetLastError());
+    return GLC_OK;
+}
+
+int launch_fit(int dtype, const void* d_values, glc_column* c, int32_t bs, int fit_form, hipStream_t s) {
+    if (c->P == 0) return GLC_OK;
+    if (bs < 32 || bs > kFitThreads) return fail(GLC_ERR_INVALID_ARG, "fit block size out of range");
+    if (fit_form != 0 && fit_form != 1) return fail(GLC_ERR_INVALID_ARG, "fit_form must be 0 or 1");
+    return dispatch_dtype(dtype, [&](auto t) {
+        using T = decltype(t);
+        return fit_form == 0 ? launch_fit_t<T, 0>(d_values, c, bs, s) : launch_fit_t<T, 1>(d_values, c, bs, s);
     });
 }
 

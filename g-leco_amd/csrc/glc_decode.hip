@@ -1,26 +1,21 @@
 // glc_decode.hip -- full decompression and pre-unpack kernels of libglc.
 //
-// Every schedule decodes "spans": a run of elements [e0, e1) inside one
-// partition.  A span is decoded by one 256-thread workgroup in three steps:
-//   1. the span's payload window (contiguous bits) is staged into LDS with
-//      coalesced 16-byte global loads (global_load_dwordx4);
-//   2. each thread extracts its elements from LDS (2-3 ds_read_b32, a
-//      funnel shift, a sign extension), e = e0 + r*256 + tid;
-//   3. FP64 prediction fma(theta1, i, theta0) + round + saturating convert +
-//      wrapping add, stored with fully coalesced global stores.
 // Schedules (the reference kernel each mode replaces in parentheses):
-//   k_decode_tiles      static grid of 2048-element tiles; a tile that spans
-//                       several partitions is split into spans
-//                       (decompressFullFile_BitPacked / _PreUnpacked,
-//                        decompression/FullDecompressionKernels.cuh:48-288)
-//   k_decode_worksteal  persistent grid; workgroups grab tiles from an atomic
-//                       counter that the last workgroup resets
-//                       (decompressFullFile_WorkStealing,
-//                        decompression/WorkStealingDecompressionKernels.cuh:332-521)
+//   k_decode_wave       one wave per 1024-element tile, static grid scheduled
+//                       by the hardware dispatcher (decompressFullFile_BitPacked /
+//                       _PreUnpacked, decompression/FullDecompressionKernels.cuh:48-288,
+//                       and decompressFullFile_WorkStealing,
+//                       decompression/WorkStealingDecompressionKernels.cuh:332-521)
+//   k_decode_steal      the same wave tiles claimed from per-XCD atomic queues
+//                       (the explicit work-stealing comparator)
 //   k_decode_parts      one workgroup per partition, or persistent workgroups
 //                       grabbing whole partitions (…_OnTheFly_Optimized_V2,
 //                        FullDecompressionKernels.cuh:293-378, and
-//                        …_WorkStealingAdvanced, WorkStealingDecompressionKernels.cuh:526-619)
+//                        …_WorkStealingAdvanced, WorkStealingDecompressionKernels.cuh:526-619).
+//                       A partition is decoded in spans of up to 2048 elements:
+//                       the span's payload window is staged into LDS with
+//                       16-byte loads, then each thread extracts, predicts and
+//                       stores its elements.
 #include <hip/hip_runtime.h>
 
 #include "glc_device.hpp"
@@ -106,77 +101,6 @@ __device__ __forceinline__ void decode_span(const DecArgs& a, const Desc& d, int
             }
         }
         __syncthreads();  // window reused by the next span
-    }
-}
-
-// Elements of [e0, e1) that no partition holds decode to 0
-// (FullDecompressionKernels.cuh:94-97).
-template <class T, int OP> __device__ __forceinline__ void zero_fill(const DecArgs& a, int64_t e0, int64_t e1) {
-    for (int64_t e = e0 + threadIdx.x; e < e1; e += kThreads) {
-        if constexpr (OP == OP_UNPACK) static_cast<int64_t*>(a.out)[e] = 0;
-        else static_cast<T*>(a.out)[e] = T(0);
-    }
-}
-
-// Decode tile [e0, e1): split into spans by partition.
-template <class T, int OP>
-__device__ __forceinline__ void decode_tile(const DecArgs& a, int64_t e0, int64_t e1, uint32_t* __restrict__ s_w) {
-    int64_t p = a.tile_part[e0 >> kIndexShift];
-    int64_t e = e0;
-    while (e < e1) {
-        // advance to the last partition with start <= e
-        while (p + 1 < a.P && a.desc[p + 1].start <= e) ++p;
-        const Desc d = a.desc[p];
-        const int64_t ds = d.start, de = ds + d.len;
-        if (e < ds) {  // gap before partition p
-            const int64_t g1 = ds < e1 ? ds : e1;
-            zero_fill<T, OP>(a, e, g1);
-            e = g1;
-            continue;
-        }
-        if (e >= de) {  // gap after partition p
-            const int64_t nx = (p + 1 < a.P) ? a.desc[p + 1].start : e1;
-            const int64_t g1 = nx < e1 ? nx : e1;
-            zero_fill<T, OP>(a, e, g1);
-            e = g1;
-            continue;
-        }
-        const int64_t s1 = de < e1 ? de : e1;
-        decode_span<T, OP>(a, d, p, e, s1, s_w);
-        e = s1;
-    }
-}
-
-template <class T, int OP>
-__global__ __launch_bounds__(kThreads) void k_decode_tiles(DecArgs a) {
-    __shared__ __align__(16) uint32_t s_w[kWinWords];
-    const int64_t e0 = static_cast<int64_t>(blockIdx.x) * kTile;
-    if (e0 >= a.n) return;
-    const int64_t e1 = (e0 + kTile < a.n) ? e0 + kTile : a.n;
-    decode_tile<T, OP>(a, e0, e1, s_w);
-}
-
-template <class T, int OP>
-__global__ __launch_bounds__(kThreads) void k_decode_worksteal(DecArgs a, int64_t ntiles) {
-    __shared__ __align__(16) uint32_t s_w[kWinWords];
-    __shared__ int s_tile;
-    for (;;) {
-        if (threadIdx.x == 0) s_tile = atomicAdd(&a.counter[0], 1);
-        __syncthreads();
-        const int64_t t = s_tile;
-        __syncthreads();
-        if (t >= ntiles) break;
-        const int64_t e0 = t * kTile;
-        const int64_t e1 = (e0 + kTile < a.n) ? e0 + kTile : a.n;
-        decode_tile<T, OP>(a, e0, e1, s_w);
-    }
-    // the last workgroup out resets the counters for the next launch
-    if (threadIdx.x == 0) {
-        __threadfence();
-        if (atomicAdd(&a.counter[1], 1) == static_cast<int>(gridDim.x) - 1) {
-            atomicExch(&a.counter[0], 0);
-            atomicExch(&a.counter[1], 0);
-        }
     }
 }
 
@@ -557,7 +481,6 @@ DecArgs make_args(const glc_column* c, void* out) {
 }
 
 template <class T, int OP> int run(const DecArgs& a, int sched, hipStream_t s) {
-    const int64_t ntiles = (a.n + kTile - 1) / kTile;
     const int64_t nwt = (a.n + kWaveTile - 1) / kWaveTile;
     switch (sched) {
         case 0:
@@ -568,16 +491,6 @@ template <class T, int OP> int run(const DecArgs& a, int sched, hipStream_t s) {
             hipLaunchKernelGGL(k_zero_queues, dim3(1), dim3(kStealQueues * kQueueStride), 0, s, a.sched);
             hipLaunchKernelGGL((k_decode_steal<T, OP>), dim3(static_cast<unsigned>((nwt + kWavesPerGroup - 1) / kWavesPerGroup)),
                                dim3(64 * kWavesPerGroup), 0, s, a, nwt);
-            break;
-        }
-        case 4:
-            hipLaunchKernelGGL((k_decode_tiles<T, OP>), dim3(static_cast<unsigned>(ntiles)), dim3(kThreads), 0, s, a);
-            break;
-        case 5: {
-            int64_t g = static_cast<int64_t>(device_cus()) * 8;
-            if (g > ntiles) g = ntiles;
-            hipLaunchKernelGGL((k_decode_worksteal<T, OP>), dim3(static_cast<unsigned>(g)), dim3(kThreads), 0, s, a,
-                               ntiles);
             break;
         }
         case 2:
@@ -620,9 +533,8 @@ int launch_decode_full(const glc_column* c, void* d_out, int mode, hipStream_t s
         case GLC_FULL_WORKSTEAL: plain = c->d_plain != nullptr; sched = 0; break;
         case GLC_FULL_WORKSTEAL_ADV: plain = c->d_plain != nullptr; sched = 3; break;
         case GLC_FULL_COOP_EXACT: plain = true; sched = 0; break;
-        // schedules kept for measurement: LDS-staged workgroup tiles (static / work stealing)
-        case kModeLdsTiles: plain = c->d_plain != nullptr; sched = 4; break;
-        case kModeLdsSteal: plain = c->d_plain != nullptr; sched = 5; break;
+        // the explicit work-stealing comparator of WorkStealingDecompressionKernels.cuh:332-521:
+        // wave tiles claimed from per-XCD atomic queues
         case kModeWaveSteal: plain = c->d_plain != nullptr; sched = 1; break;
         default: return fail(GLC_ERR_INVALID_ARG, "bad full-decompress mode");
     }

@@ -30,9 +30,8 @@ constexpr int kPayloadPadWords = 192;           // zero words after the payload:
 constexpr int kDecodeThreads = 256;             // 4 wave64 per workgroup
 constexpr int kDecodeTile = 2048;               // elements per workgroup tile (8 per thread)
 constexpr int kMaxTileParts = 64;               // partitions of one tile staged in LDS (slow path)
-constexpr int kModeLdsTiles = 100;              // extra full-decode schedules (measurement): LDS workgroup tiles
-constexpr int kModeLdsSteal = 101;
-constexpr int kModeWaveSteal = 102;  // wave tiles claimed from per-XCD atomic queues (measurement)              //   ... and their persistent work-stealing form
+constexpr int kModeWaveSteal = 102;             // full-decode mode: wave tiles claimed from per-XCD atomic
+                                                // queues (the explicit work-stealing comparator)
 
 enum { MODEL_CONSTANT = 0, MODEL_LINEAR = 1, MODEL_POLY2 = 2, MODEL_POLY3 = 3, MODEL_DIRECT_COPY = 4 };
 
@@ -174,6 +173,7 @@ struct glc_column {
     int32_t fixed_size = 0;
     int32_t max_bits = 0;
     int32_t fit_block = 0;
+    int32_t fit_form = 0;           // contraction form of the fit (glc_opts.fit_form)
     int32_t has_poly = 0;
     // arena
     void* meta_arena = nullptr;
@@ -208,6 +208,15 @@ int fail(int code, const std::string& msg);
             return ::glc::fail(GLC_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));     \
     } while (0)
 
+// GLC_HIP inside a function that owns partly built state: the error goes
+// through the function's local `bail` (which releases that state) first.
+#define GLC_HIP_BAIL(call)                                                                          \
+    do {                                                                                            \
+        hipError_t e_ = (call);                                                                     \
+        if (e_ != hipSuccess)                                                                       \
+            return bail(::glc::fail(GLC_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_))); \
+    } while (0)
+
 inline int elem_size_of(int dtype) { return (dtype == GLC_I32 || dtype == GLC_U32) ? 4 : 8; }
 inline bool valid_dtype(int dtype) { return dtype >= GLC_I32 && dtype <= GLC_U64; }
 
@@ -223,7 +232,7 @@ inline int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
 
 // ---- launchers implemented in the .hip translation units -----------------
 // compress side (glc_compress.hip)
-int launch_fit(int dtype, const void* d_values, glc_column* c, int32_t bs, hipStream_t s);
+int launch_fit(int dtype, const void* d_values, glc_column* c, int32_t bs, int fit_form, hipStream_t s);
 int launch_scan_offsets(glc_column* c, int64_t* d_total, hipStream_t s);
 int launch_pack(int dtype, const void* d_values, glc_column* c, hipStream_t s);
 int launch_build_index(glc_column* c, IndexSummary* d_sum, hipStream_t s);

@@ -46,7 +46,7 @@ class ColumnInfo(C.Structure):
 
 class Opts(C.Structure):
     _fields_ = [("partitioner", C.c_int32), ("partition_size", C.c_int32), ("v6_base_size", C.c_int32),
-                ("reserved", C.c_int32 * 5)]
+                ("fit_form", C.c_int32), ("reserved", C.c_int32 * 4)]
 
 
 _lib = None
@@ -334,20 +334,23 @@ class Column:
             lib().glc_free_blob(blob)
 
 
-def _opts(partitioner: int, partition_size: int, base_size: int) -> Opts:
+def _opts(partitioner: int, partition_size: int, base_size: int, fit_form: int = 0) -> Opts:
     o = Opts()
     o.partitioner = partitioner
     o.partition_size = partition_size
     o.v6_base_size = base_size
+    o.fit_form = fit_form
     return o
 
 
 def compress(values, partitioner: int = PART_FIXED, partition_size: int = 0, base_size: int = 0,
-             stream=None) -> Column:
-    """LeCoGPU<T>::compress on a device tensor (api/G-LeCo.cuh:349-614)."""
+             stream=None, fit_form: int = 0) -> Column:
+    """LeCoGPU<T>::compress on a device tensor (api/G-LeCo.cuh:349-614).
+    fit_form selects the FP64 contraction of the fit (0 = fused, the default;
+    1 = every product rounded on its own)."""
     dt = dtype_of(values)
     h = C.c_void_p()
-    o = _opts(partitioner, partition_size, base_size)
+    o = _opts(partitioner, partition_size, base_size, fit_form)
     n = values.numel()
     _check(lib().glc_compress(dt, _devptr(values) if n else None, n, C.byref(o), _stream(stream), C.byref(h)))
     return Column(h, dt)

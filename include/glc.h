@@ -55,7 +55,11 @@ typedef struct {
     int32_t partitioner;        /* glc_partitioner */
     int32_t partition_size;     /* FIXED: S (default 4096); SPLITMERGE: min partition size (default 128) */
     int32_t v6_base_size;       /* V6: base size (default 2048, G:386) */
-    int32_t reserved[5];
+    int32_t fit_form;           /* FP64 contraction of the fit, compression/CompressionKernels.cuh:113-158:
+                                 * 0 = fma in sum_xx, sum_xy, det, numerator and theta0 (default; nvcc's
+                                 * --fmad=true contraction of the source as written), 1 = no fma there.
+                                 * The prediction is fma(theta1, i, theta0) in both. */
+    int32_t reserved[4];
 } glc_opts;
 
 typedef enum {
@@ -204,7 +208,12 @@ int glc_hostio_config(int32_t* slots, int64_t* slot_bytes, int32_t* threads);
  * types and lengths, each cut into fixed partitions of one size.  Every step
  * runs as a fixed number of kernel launches whatever the column count, needs
  * no host synchronisation and no allocation after glc_batch_create, and can
- * therefore be recorded once into a hipGraph and replayed. */
+ * therefore be recorded once into a hipGraph and replayed.  Calls on one batch
+ * may pass different streams: each call's device work is ordered after the
+ * batch's previous compress and decompress (event waits), so a decompress
+ * never reads a payload that a compress on another stream is still writing.
+ * A batch is not thread-safe: calls on one batch come from one thread at a
+ * time. */
 typedef struct glc_batch glc_batch;
 /* Plan and allocate a batch over device columns values[i] (n[i] elements of
  * dtypes[i]).  The payload reserve per column is its worst case (33 bits per

@@ -124,3 +124,44 @@ def test_host_copy_argument_checks_need_no_device():
     assert L.glc_copy_to_device(C.addressof(buf), None, 16, None) == -1
     with pytest.raises(ValueError):
         glc.copy_to_host(np.zeros((4, 4), np.int64)[:, 1], None)     # not contiguous
+
+
+def v4_blob(blob: bytes) -> bytes:
+    """The version-4 form of a v5 blob (api/G-LeCo.cuh:1029-1049): version
+    field 4, no 8-byte alignment of the total size (the blob ends at the
+    payload's last byte), header checksum resealed."""
+    delta_off, _, delta_bytes = struct.unpack_from("<3Q", blob, 72)
+    b = _corrupt(blob, 4, struct.pack("<I", 4))[:delta_off + delta_bytes]
+    return _reseal(b)
+
+
+def unaligned_v4_case(gen, n0):
+    """(values, oracle column, v4 blob) for the first length >= n0 whose
+    blob does not end on an 8-byte boundary."""
+    gens = {"c1": W.c1_int32, "c2": lambda n: W.c2_uint64(n, nseg=8), "c3": W.c3_int64,
+            "u32": lambda n: (W.c1_int32(n).astype(np.int64) + (1 << 31)).astype(np.uint32)}
+    for n in range(n0, n0 + 64):
+        v = gens[gen](n)
+        col = O.compress(v, O.PART_FIXED, 1000)
+        b4 = v4_blob(col.blob)
+        if len(b4) % 8:
+            return v, col, b4
+    raise AssertionError("no unaligned length found")
+
+
+@pytest.mark.parametrize("gen,dt", [("c1", glc.I32), ("c2", glc.U64), ("c3", glc.I64)])
+def test_v4_blob_direct_access_cpu(gen, dt):
+    v, col, b4 = unaligned_v4_case(gen, 20001)
+    n = v.size
+    assert struct.unpack_from("<I", b4, 4)[0] == 4
+    assert len(b4) % 8 != 0 and len(b4) < len(col.blob)  # the v5 tail padding is gone
+    h = glc.DirectHandle(b4, dt, copy_to_device=False)
+    pos = np.concatenate([np.arange(0, n, 7), [n, -1]]).astype(np.int32)
+    assert np.array_equal(h.random_access_cpu(pos), O.random_access(col, pos.astype(np.int64)))
+    h.close()
+    # a v5 blob cut the same way is short of its aligned size and is refused
+    if len(b4) % 8:
+        b5 = _reseal(col.blob[:len(b4)])
+        with pytest.raises(glc.GlcError) as e:
+            glc.deserialize(b5, dt, stream=0)
+        assert e.value.code == -4

@@ -161,3 +161,34 @@ def test_batch_mixed_partition_kinds(gpu):
     for k, (v, o) in enumerate(zip(vals, outs)):
         assert np.array_equal(o.cpu().numpy(), v), k
         check_view(b.column(k), v, 1024)
+
+
+def test_batch_calls_on_different_streams(gpu):
+    """Compress on one stream and decompress on another (and back): each call
+    is ordered after the batch's previous compress and decompress, so every
+    column equals the oracle's compression and decodes to its input."""
+    lens = [5000, 70000, 1024, 300001, 4097, 20000]
+    kinds = [glc.I64, glc.I32, glc.U64, glc.U32, glc.I64, glc.I32]
+    vals = make_columns(41, lens, kinds)
+    dev = [torch.from_numpy(v).cuda() for v in vals]
+    b = glc.Batch(dev, 1000)
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    torch.cuda.synchronize()
+    for rnd in range(3):
+        b.compress(stream=s1)
+        outs = [torch.full_like(t, 7) for t in dev]
+        torch.cuda.synchronize()
+        b.decompress(outs, stream=s2)           # must wait for the compress on s1
+        s2.synchronize()
+        for v, o in zip(vals, outs):
+            assert np.array_equal(o.cpu().numpy(), v), rnd
+        outs2 = [torch.full_like(t, 9) for t in dev]
+        torch.cuda.synchronize()
+        b.run_graph(glc.BATCH_DECOMPRESS, outs2, stream=s1)  # after the decode on s2
+        b.decompress(outs, stream=s2)           # rewrites the output table after the graph's decode
+        s1.synchronize()
+        s2.synchronize()
+        for v, o, o2 in zip(vals, outs, outs2):
+            assert np.array_equal(o.cpu().numpy(), v) and np.array_equal(o2.cpu().numpy(), v), rnd
+    for i, v in enumerate(vals):
+        check_view(b.column(i), v, 1000)

@@ -23,4 +23,6 @@ def test_leco_gpu_dropin_app(gpu):
     sys.stderr.write(r.stderr[-2000:])
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
     assert "ALL PASSED" in r.stdout
+    # analyzeRandomAccessPerformance is called as app/Main.cu:151 calls it (fixed partitions)
+    assert r.stdout.count("--- Random Access Performance Analysis ---") == 5
     assert r.stdout.count("PASSED") >= 5 * 3 * 15

@@ -47,10 +47,9 @@ def assert_same_column(col, ref, check_fit_block=True):
 
 def check_decoders(col, v, ref, fixed_size=None, q=4000, seed=0):
     n = v.size
-    # 100, 101, 102: the LDS-tile, LDS work-stealing and atomic-claim wave
-    # schedules kept for measurement
-    for mode in (glc.FULL_AUTO, glc.FULL_BITPACKED, glc.FULL_V2, glc.FULL_WORKSTEAL, glc.FULL_WORKSTEAL_ADV,
-                 100, 101, 102):
+    # 102: wave tiles claimed from per-XCD atomic queues (the explicit
+    # work-stealing comparator)
+    for mode in (glc.FULL_AUTO, glc.FULL_BITPACKED, glc.FULL_V2, glc.FULL_WORKSTEAL, glc.FULL_WORKSTEAL_ADV, 102):
         out = host(col.decompress_full(mode))
         assert np.array_equal(out, v), f"full mode {mode}"
     rng = np.random.default_rng(seed)
@@ -175,6 +174,10 @@ def test_gpu_splitmerge_boundaries(gpu, seg, monkeypatch):
              (jumpy_runs(1000, 12, 41, 10), 3, O.PART_SPLITMERGE_EXACT),
              (jumpy_runs(999, 13, 23, 5), 2, O.PART_SPLITMERGE_EXACT),
              (jumpy_runs(4100, 14, 61, 20), 1, O.PART_SPLITMERGE_EXACT),
+             # more than 65535 block starts: more than 256 prefix chunks, so one
+             # thread of the chunk scan carries several i128 chunk totals
+             (jumpy_runs(1 << 17, 15, 3000, 4), 1, O.PART_SPLITMERGE_EXACT),
+             (jumpy_runs((1 << 18) + 77, 16, 5000, 6), 2, O.PART_SPLITMERGE_EXACT),
              (np.array([5], dtype=np.int64), 128, O.PART_CPU)]
     for v, m, mode in cases:
         print("split/merge case", v.dtype, v.size, m, "segment", seg, flush=True)
@@ -379,3 +382,52 @@ def test_parity_mixed_partition_kinds(gpu, dt, S):
     print(f"mixed kinds dtype {np.dtype(dt).name} S {S}", flush=True)
     v = mixed_kind_column(dt, n, S, seed=S * 7 + np.dtype(dt).itemsize)
     full_case(v, "fixed", S, fixed_size=S)
+
+
+@pytest.mark.parametrize("gen", ["c1", "u32", "c2", "c3"])
+def test_v4_blob_reads_like_v5(gpu, gen):
+    """A version-4 blob (api/G-LeCo.cuh:1029-1049: no tail alignment) decodes
+    through deserialize, pre-unpacked deserialize and the direct handle on
+    the GPU and the CPU, equal to the oracle."""
+    from test_abi import unaligned_v4_case
+    v, ref, b4 = unaligned_v4_case(gen, 30001)
+    n = v.size
+    assert len(b4) % 8
+    dt = DT_OF[v.dtype]
+    d = glc.deserialize(b4, dt)
+    assert_same_column(d, ref, check_fit_block=False)
+    assert np.array_equal(host(d.decompress_full()), v)
+    assert d.serialize() == ref.blob  # written back as v5
+    d2 = glc.deserialize(b4, dt, preunpack=True)
+    assert np.array_equal(host(d2.decompress_full()), v)
+    pos = np.random.default_rng(9).integers(-2, n + 2, 5000).astype(np.int32)
+    want = O.random_access(ref, pos.astype(np.int64))
+    assert np.array_equal(host(d2.random_access(to_dev(pos), glc.RA_PREUNPACKED)), want)
+    h = glc.DirectHandle(b4, dt)
+    assert np.array_equal(host(h.random_access_gpu(to_dev(pos))), want)
+    assert np.array_equal(h.random_access_cpu(pos), want)
+
+
+@pytest.mark.parametrize("gen,part", [("u64", "v6"), ("i64", "fixed"), ("i32", "fixed"), ("u32", "v6")])
+def test_parity_fit_form_1(gpu, gen, part):
+    """glc_opts.fit_form = 1 (no fma in the fit's sums, determinant, numerator
+    and intercept; compression/CompressionKernels.cuh:113-158 as written
+    without contraction) against the oracle's form 1, bit for bit, on values
+    whose sums are inexact in FP64; the two forms give different theta bits
+    there and both round-trip."""
+    n = 200_000
+    v = GENS[gen](n)
+    gp, op = PART[part]
+    size = 4096 if part == "fixed" else 0
+    col = glc.compress(to_dev(v), gp, size, 0, fit_form=1)
+    ref = O.compress(v, op, 4096, 2048, fit_form=1)
+    assert_same_column(col, ref)
+    assert col.serialize() == ref.blob
+    assert np.array_equal(host(col.decompress_full()), v)
+    pos = np.random.default_rng(3).integers(0, n, 4000).astype(np.int32)
+    assert np.array_equal(host(col.random_access(to_dev(pos))), v[pos])
+    ref0 = O.compress(v, op, 4096, 2048, fit_form=0)
+    if gen in ("u64", "i64"):  # sums above 2^53: the contraction shows in theta
+        assert not np.array_equal(ref0.params.view(np.uint64), ref.params.view(np.uint64))
+    with pytest.raises(glc.GlcError):
+        glc.compress(to_dev(v[:1000]), gp, size, 0, fit_form=2)
