@@ -239,7 +239,7 @@ int compress_impl(int dtype, const void* d_values, int64_t n, int partitioner, i
             int64_t* tc = reinterpret_cast<int64_t*>(t + 2 * align_up(cap * 4, 256));
             float* tv = reinterpret_cast<float*>(t + 2 * align_up(cap * 4, 256) + align_up((nvb + 1) * 8, 256));
             float* tt = reinterpret_cast<float*>(reinterpret_cast<char*>(tv) + align_up(nvb * 4, 256));
-            rc = launch_v6(dtype, d_values, n, base, ts, te, cap, tc, tv, tt, s);
+            rc = launch_v6(dtype, d_values, n, base, ts, te, cap, tc, tc + 1, tv, tt, s);
             int64_t P = 0;
             int32_t first[2] = {0, 0};
             if (!rc && hipMemcpyAsync(&P, tc, 8, hipMemcpyDeviceToHost, s) != hipSuccess) rc = GLC_ERR_HIP;
@@ -395,6 +395,48 @@ int fail(int code, const std::string& msg) {
 
 using namespace glc;
 
+// ---- allocation-free V6 partitioning -----------------------------------------
+// glc_partition_v6_noalloc keeps the V6 kernels' scratch (block statistics,
+// the three thresholds, per-block partition bases) in this file-scope device
+// array instead of memory it allocates.  Each device holds its own copy at
+// its own address until the process ends; host code reaches it only through
+// the address hipGetSymbolAddress returns for the current device.
+constexpr int64_t kNoAllocMaxBlocks = 1 << 17;  // n < 2^31 at the default base 2048
+constexpr int64_t kNoAllocVarOff = 0;
+constexpr int64_t kNoAllocThrOff = kNoAllocMaxBlocks * 4;
+constexpr int64_t kNoAllocBaseOff = kNoAllocThrOff + 256;
+constexpr int64_t kNoAllocScratchBytes = kNoAllocBaseOff + kNoAllocMaxBlocks * 8;  // 1.5 MiB + 256 B
+static __device__ __align__(256) char scratch[kNoAllocScratchBytes];
+
+namespace {
+// The current device's address of `scratch`, looked up once per device.
+int noalloc_scratch(char** out) {
+    int dev = 0;
+    GLC_HIP(hipGetDevice(&dev));
+    static std::mutex mu;
+    static std::map<int, char*>* table = new std::map<int, char*>();  // used until exit
+    std::lock_guard<std::mutex> g(mu);
+    auto it = table->find(dev);
+    if (it != table->end()) {
+        *out = it->second;
+        return GLC_OK;
+    }
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+        (void)hipGetLastError();
+        return fail(GLC_ERR_HIP, std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+    }
+    (*table)[dev] = static_cast<char*>(p);
+    *out = static_cast<char*>(p);
+    return GLC_OK;
+}
+
+__global__ void k_store_i64(int64_t* p, int64_t v) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+}  // namespace
+
 // ===========================================================================
 // C-ABI
 // ===========================================================================
@@ -509,6 +551,36 @@ int glc_partition(int dtype, const void* d_values, int64_t n, const glc_opts* op
     if (c->P <= cap) rc = glc_column_copy_meta(c, h_start, h_end, nullptr, nullptr, nullptr, nullptr, nullptr);
     destroy(c);
     return rc;
+}
+
+int glc_partition_v6_noalloc(int dtype, const void* d_values, int64_t n, int32_t base_size, int32_t* d_start,
+                             int32_t* d_end, int64_t cap, int64_t* d_num_partitions, void* stream) {
+    if (!valid_dtype(dtype)) return fail(GLC_ERR_INVALID_ARG, "bad dtype");
+    if (n < 0 || n > 0x7FFFFFFFLL) return fail(GLC_ERR_INVALID_ARG, "n must be in [0, 2^31) (int32 indices, H21)");
+    if (!d_num_partitions) return fail(GLC_ERR_INVALID_ARG, "num_partitions is NULL");
+    const int32_t base = base_size > 0 ? base_size : 2048;
+    int64_t need = 0, nvb = 0;
+    int rc = v6_capacity(n, base, &need, &nvb);
+    if (rc) return rc;
+    if (nvb > kNoAllocMaxBlocks)
+        return fail(GLC_ERR_INVALID_ARG, "column has " + std::to_string(nvb) + " V6 blocks of " +
+                                             std::to_string(8LL * base) + " values; the allocation-free scratch holds " +
+                                             std::to_string(kNoAllocMaxBlocks));
+    if (n > 0 && (!d_values || !d_start || !d_end)) return fail(GLC_ERR_INVALID_ARG, "NULL values or partition array");
+    if (n > 0 && cap < need)
+        return fail(GLC_ERR_INVALID_ARG, "partition arrays hold " + std::to_string(cap) + " entries; V6 may emit up to " +
+                                             std::to_string(need) + " (16 per block of 8*base values)");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (n == 0) {
+        hipLaunchKernelGGL(k_store_i64, dim3(1), dim3(64), 0, s, d_num_partitions, int64_t(0));
+        GLC_HIP(hipGetLastError());
+        return GLC_OK;
+    }
+    char* sc = nullptr;
+    if ((rc = noalloc_scratch(&sc))) return rc;
+    return launch_v6(dtype, d_values, n, base, d_start, d_end, cap, d_num_partitions,
+                     reinterpret_cast<int64_t*>(sc + kNoAllocBaseOff), reinterpret_cast<float*>(sc + kNoAllocVarOff),
+                     reinterpret_cast<float*>(sc + kNoAllocThrOff), s);
 }
 
 static bool fixed_compatible(const glc_column* c, int32_t S) {

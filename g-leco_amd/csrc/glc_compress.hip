@@ -1232,7 +1232,7 @@ int v6_capacity(int64_t n, int32_t base, int64_t* cap, int64_t* nvb) {
 }
 
 int launch_v6(int dtype, const void* d_values, int64_t n, int32_t base, int32_t* d_start, int32_t* d_end,
-              int64_t cap, int64_t* d_count, float* d_var, float* d_thr, hipStream_t s) {
+              int64_t cap, int64_t* d_total, int64_t* d_base, float* d_var, float* d_thr, hipStream_t s) {
     int64_t need = 0, nvb = 0;
     int rc = v6_capacity(n, base, &need, &nvb);
     if (rc) return rc;
@@ -1250,9 +1250,7 @@ int launch_v6(int dtype, const void* d_values, int64_t n, int32_t base, int32_t*
     hipLaunchKernelGGL(k_v6_select, dim3(1), dim3(1024), 0, s, d_var, nvb, d_thr);
     GLC_HIP(hipGetLastError());
     V6Count f{d_var, d_thr, vbs, n, base};
-    // per-block partition base index (exclusive scan) stored after the counts slot
-    int64_t* d_base = d_count + 1;
-    rc = launch_scan(f, nvb, d_base, d_count, s);
+    rc = launch_scan(f, nvb, d_base, d_total, s);  // per-block partition bases, P
     if (rc) return rc;
     GLC_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_v6_emit, dim3(grid_for(nvb, 256)), dim3(256), 0, s, f, nvb, d_base, d_start, d_end);

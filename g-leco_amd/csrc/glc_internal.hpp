@@ -237,8 +237,11 @@ int launch_scan_offsets(glc_column* c, int64_t* d_total, hipStream_t s);
 int launch_pack(int dtype, const void* d_values, glc_column* c, hipStream_t s);
 int launch_build_index(glc_column* c, IndexSummary* d_sum, hipStream_t s);
 int launch_build_tinfo(glc_column* c, hipStream_t s);  // after a valid index
+// V6 boundaries into d_start/d_end[cap]; *d_total receives P, d_base[nvb]
+// the per-block partition bases; d_var[nvb], d_thr[3] are scratch.  Kernel
+// launches only (no memset, no copy, no host wait).
 int launch_v6(int dtype, const void* d_values, int64_t n, int32_t base, int32_t* d_start, int32_t* d_end,
-              int64_t cap, int64_t* d_count, float* d_var, float* d_thr, hipStream_t s);
+              int64_t cap, int64_t* d_total, int64_t* d_base, float* d_var, float* d_thr, hipStream_t s);
 int v6_capacity(int64_t n, int32_t base, int64_t* cap, int64_t* nvb);
 int launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s);
 // decode side (glc_decode.hip)

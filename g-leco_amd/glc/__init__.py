@@ -64,6 +64,7 @@ SIGNATURES = [
     ("glc_column_copy_meta", _int, [_vp] * 8),
     ("glc_column_copy_payload", _int, [_vp, _vp, _i64]),
     ("glc_partition", _int, [_int, _vp, _i64, _vp, _vp, _vp, _vp, _i64, _vp]),
+    ("glc_partition_v6_noalloc", _int, [_int, _vp, _i64, _i32, _vp, _vp, _i64, _vp, _vp]),
     ("glc_decompress_full", _int, [_vp, _vp, _int, _vp]),
     ("glc_decompress_full_fixed", _int, [_vp, _vp, _i32, _vp]),
     ("glc_random_access", _int, [_vp, _vp, _i64, _vp, _int, _vp]),
@@ -377,6 +378,14 @@ def partition(values, partitioner: int, partition_size: int = 0, base_size: int 
     _check(lib().glc_partition(dt, _devptr(values), values.numel(), C.byref(o), _stream(stream),
                                st.ctypes.data_as(C.c_void_p), en.ctypes.data_as(C.c_void_p), P.value, C.byref(P)))
     return st, en
+
+
+def partition_v6_noalloc(values, d_start, d_end, d_count, base_size: int = 0, stream=None) -> None:
+    """V6 boundaries into caller-owned device tensors (int32 d_start/d_end,
+    int64 d_count[1]); enqueued on the stream, nothing allocated."""
+    _check(lib().glc_partition_v6_noalloc(dtype_of(values), _devptr(values) if values.numel() else None,
+                                          values.numel(), base_size, _devptr(d_start), _devptr(d_end),
+                                          d_start.numel(), _devptr(d_count), _stream(stream)))
 
 
 def deserialize(blob: bytes, dtype: int, preunpack: bool = False, stream=None) -> Column:

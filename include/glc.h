@@ -136,6 +136,21 @@ int glc_column_copy_payload(const glc_column* col, uint32_t* h_words, int64_t n_
  * (host arrays of capacity `cap`; *num_partitions receives P even if cap is small). */
 int glc_partition(int dtype, const void* d_values, int64_t n, const glc_opts* opts, void* stream, int32_t* h_start,
                   int32_t* h_end, int64_t cap, int64_t* num_partitions);
+/* Allocation-free V6 partitioning.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under hipGraph
+ * capture, serving loops, harnesses that forbid hipMalloc).  It gives the
+ * boundaries glc_partition gives with GLC_PART_V6 and base_size (<= 0 means
+ * 2048), written to the device arrays d_start/d_end (capacity cap, at least
+ * 16 * ceil(n / (8 * base_size))) and P to the device int64
+ * *d_num_partitions.  Kernel launches only on `stream`: no allocation, no
+ * memset, no copy, no host wait.  The kernels' scratch (1.5 MiB) is a
+ * static device array of the library, one per device, kept until the process
+ * ends; it limits a column to 131072 blocks of 8 * base_size values (any
+ * n < 2^31 at the default base) and larger requests fail with
+ * GLC_ERR_INVALID_ARG.  Calls on one device share that scratch: calls that
+ * overlap on the device (different streams, not ordered) race. */
+int glc_partition_v6_noalloc(int dtype, const void* d_values, int64_t n, int32_t base_size, int32_t* d_start,
+                             int32_t* d_end, int64_t cap, int64_t* d_num_partitions, void* stream);
 
 /* ---- full decompression ---------------------------------------------------
  * d_out: device array of total_values values.  Asynchronous on `stream`.   */
