@@ -28,15 +28,9 @@
 namespace glc {
 namespace {
 
-template <class T> T round_cast(double r) {
-    r = std::round(r);
-    if (r != r) return T(0);
-    const double lo = std::is_signed<T>::value ? static_cast<double>(std::numeric_limits<T>::min()) : 0.0;
-    const double hi = static_cast<double>(std::numeric_limits<T>::max());
-    if (r <= lo) return std::numeric_limits<T>::min();
-    if (r >= hi) return std::numeric_limits<T>::max();
-    return static_cast<T>(r);
-}
+// :106 static_cast<T>(std::round(predicted)), as the reference's x86-64
+// host build converts out-of-range values (x86_cast).
+template <class T> T round_cast(double r) { return x86_cast<T>(std::round(r)); }
 
 template <class T> int64_t delta_of(T actual, T pred) {  // core/MathHelpers.cuh:22-53
     if (std::is_signed<T>::value)

@@ -17,6 +17,7 @@
 
 #include <cstdint>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "glc.h"
@@ -229,6 +230,34 @@ void bind_arena(glc_column* c, char* base, const ArenaLayout& L);
 IndexSummary* summary_ptr(glc_column* c);
 int32_t fit_block_size(int64_t len0);
 inline int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
+
+// static_cast<T>(r) of an integral-valued double as the reference's host
+// split/merge cost model gets it (compression/CpuPartitioner.cuh:106 writes
+// static_cast<T>(std::round(pred)); the reference builds host code with g++
+// for x86-64 without -march, i.e. SSE2 cvttsd2si).  NaN and out-of-range
+// input give the conversion width's "integer indefinite" 0x80..0; uint32
+// is the low half of the 64-bit conversion; uint64 at or above 2^63 converts
+// r - 2^63 and flips bit 63.  Used by the host partitioner and the GPU
+// split/merge, which share that cost model; every decoder and the encoder
+// use CUDA's saturating conversion instead (sat_cast).
+__host__ __device__ inline int64_t x86_cvt64(double r) {
+    if (!(r >= -9223372036854775808.0 && r < 9223372036854775808.0)) return INT64_MIN;
+    return static_cast<int64_t>(r);
+}
+template <class T> __host__ __device__ inline T x86_cast(double r) {
+    if constexpr (sizeof(T) == 4 && std::is_signed<T>::value) {
+        if (!(r > -2147483649.0 && r < 2147483648.0)) return static_cast<T>(INT32_MIN);
+        return static_cast<T>(static_cast<int32_t>(r));
+    } else if constexpr (sizeof(T) == 4) {
+        return static_cast<T>(static_cast<uint64_t>(x86_cvt64(r)));
+    } else if constexpr (std::is_signed<T>::value) {
+        return static_cast<T>(x86_cvt64(r));
+    } else {
+        if (r >= 9223372036854775808.0)
+            return static_cast<T>(static_cast<uint64_t>(x86_cvt64(r - 9223372036854775808.0)) ^ (1ULL << 63));
+        return static_cast<T>(static_cast<uint64_t>(x86_cvt64(r)));
+    }
+}
 
 // ---- launchers implemented in the .hip translation units -----------------
 // compress side (glc_compress.hip)

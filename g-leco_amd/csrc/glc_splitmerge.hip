@@ -124,12 +124,12 @@ __device__ void sm_theta(i128 Sy, i128 Sxy, int64_t n, double& t0, double& t1) {
 
 template <class T> __device__ __forceinline__ int64_t abs_err(T v, double t0, double t1, int64_t i) {
     const double pr = t0 + t1 * static_cast<double>(i);  // no contraction (-ffp-contract=off)
-    const int64_t d = calc_delta<T>(v, sat_cast<T>(__builtin_round(pr)));
+    const int64_t d = calc_delta<T>(v, x86_cast<T>(__builtin_round(pr)));  // the host cost model's conversion
     return d < 0 ? static_cast<int64_t>(0ULL - static_cast<uint64_t>(d)) : d;
 }
 
 // Residual magnitude in FP64 when it is exact.  For values v and rounded
-// predictions p of magnitude <= 2^50 (inside T for 32-bit types), sat_cast is
+// predictions p of magnitude <= 2^50 (inside T for 32-bit types), x86_cast is
 // the identity on p, calc_delta is v - p without wrap or saturation, and the
 // FP64 difference of two such integers is exact; so fabs(v - round(pr)) is
 // abs_err's result.  The prediction t0 + t1 * i is monotonic in i, so its two

@@ -76,6 +76,40 @@ template <class T> inline T sat_cast(double r) {
 // round-half-away-from-zero then saturate: (T)round(pred).
 template <class T> inline T pred_to_T(double pred) { return sat_cast<T>(std::round(pred)); }
 
+// static_cast<T>(r) of an integral-valued double as the reference's host code
+// gets it: compression/CpuPartitioner.cuh:106 writes static_cast<T>(
+// std::round(pred)), which is undefined behaviour out of range; the
+// reference Makefile builds host code with g++ for x86-64 without -march,
+// where the conversion is SSE2 cvttsd2si.  That instruction returns the
+// "integer indefinite" value 0x80..0 of its width for NaN and out-of-range
+// input; g++ converts to uint32 through the 64-bit instruction and keeps the
+// low half, and to uint64 through the 64-bit instruction below 2^63 and
+// through r - 2^63 with bit 63 flipped at or above it.  So for an unsigned
+// column a prediction rounding to -1 becomes the type's maximum and the
+// candidate's residual (and bit width) is large, where CUDA's saturating
+// device conversion would give 0.
+// Pinned by tests/test_oracle_x86_cast.py (Intel SDM rules as a table, and
+// g++'s own static_cast compiled in the test when g++ is present).
+inline int64_t x86_cvt64(double r) {
+    if (!(r >= -9223372036854775808.0 && r < 9223372036854775808.0)) return INT64_MIN;
+    return static_cast<int64_t>(r);
+}
+template <class T> inline T x86_cast(double r) {
+    if (std::is_same<T, int32_t>::value) {
+        if (!(r > -2147483649.0 && r < 2147483648.0)) return static_cast<T>(INT32_MIN);
+        return static_cast<T>(static_cast<int32_t>(r));
+    } else if (std::is_same<T, uint32_t>::value) {
+        return static_cast<T>(static_cast<uint64_t>(x86_cvt64(r)));
+    } else if (std::is_same<T, int64_t>::value) {
+        return static_cast<T>(x86_cvt64(r));
+    } else {
+        if (r >= 9223372036854775808.0)
+            return static_cast<T>(static_cast<uint64_t>(x86_cvt64(r - 9223372036854775808.0)) ^ (1ULL << 63));
+        return static_cast<T>(static_cast<uint64_t>(x86_cvt64(r)));
+    }
+}
+template <class T> inline T host_pred_to_T(double pred) { return x86_cast<T>(std::round(pred)); }
+
 // core/MathHelpers.cuh:22-53.  Signed subtraction wraps like the GPU's.
 // Pinned by tests/test_oracle_scalar.py (brute force + saturation table).
 template <class T> inline int64_t calc_delta(T actual, T pred) {
@@ -580,7 +614,7 @@ template <class T> struct CpuPartitioner {
         mx = 0;
         for (int32_t i = 0; i < n_; ++i) {
             double pr = t0 + t1 * static_cast<double>(i);
-            T p = pred_to_T<T>(pr);
+            T p = host_pred_to_T<T>(pr);  // :106, host conversion
             int64_t d = calc_delta<T>(v[s + i], p);
             int64_t a = d < 0 ? static_cast<int64_t>(0ULL - static_cast<uint64_t>(d)) : d;
             mx = std::max(mx, a);
@@ -717,7 +751,7 @@ template <class T> struct ExactSplitMerge {
         int64_t mx = 0;
         for (int64_t i = 0; i < len; ++i) {
             const double pr = t0 + t1 * static_cast<double>(i);
-            const int64_t d = calc_delta<T>(v[s + i], pred_to_T<T>(pr));
+            const int64_t d = calc_delta<T>(v[s + i], host_pred_to_T<T>(pr));
             const int64_t a = d < 0 ? static_cast<int64_t>(0ULL - static_cast<uint64_t>(d)) : d;
             mx = std::max(mx, a);
         }
@@ -1044,6 +1078,13 @@ uint64_t orc_pred_to_T(int dtype, double pred) {
         using T = decltype(t);
         T r = pred_to_T<T>(pred);
         return static_cast<uint64_t>(static_cast<typename std::make_unsigned<T>::type>(r));
+    });
+}
+uint64_t orc_x86_cast(int dtype, double r) {
+    return dispatch(dtype, [&](auto t) -> uint64_t {
+        using T = decltype(t);
+        T x = x86_cast<T>(r);
+        return static_cast<uint64_t>(static_cast<typename std::make_unsigned<T>::type>(x));
     });
 }
 uint64_t orc_decode_pred(int dtype, int32_t type, const double* prm, int64_t i) {

@@ -57,6 +57,8 @@ def lib() -> C.CDLL:
         L.orc_apply_delta.argtypes = [C.c_int, C.c_uint64, i64]
         L.orc_pred_to_T.restype = C.c_uint64
         L.orc_pred_to_T.argtypes = [C.c_int, C.c_double]
+        L.orc_x86_cast.restype = C.c_uint64
+        L.orc_x86_cast.argtypes = [C.c_int, C.c_double]
         L.orc_decode_pred.restype = C.c_uint64
         L.orc_decode_pred.argtypes = [C.c_int, i32, vp, i64]
         L.orc_bits_for_max_error.restype = i32
@@ -212,6 +214,11 @@ def apply_delta(dtype: int, pred: int, delta: int) -> int:
 def pred_to_T(dtype: int, pred: float) -> int:
     """(T)round(pred) with CUDA saturation; returned as the unsigned bit pattern."""
     return lib().orc_pred_to_T(dtype, pred)
+
+
+def x86_cast(dtype: int, r: float) -> int:
+    """static_cast<T>(r) as g++ emits it for x86-64 (SSE2), as an unsigned bit pattern of T."""
+    return lib().orc_x86_cast(dtype, r)
 
 
 def decode_pred(dtype: int, model_type: int, params, i: int) -> int:

@@ -189,6 +189,34 @@ def test_gpu_splitmerge_boundaries(gpu, seg, monkeypatch):
             assert np.array_equal(ex.start, ref.start)
 
 
+def hockey_sticks(n, dtype, seed):
+    """Unsigned ramps down to 0 followed by runs of zeros: fitted lines of
+    split/merge candidates that cover a ramp's foot predict values below 0,
+    which the host cost model converts the x86 way (a rounded -1 becomes
+    T's maximum for uint32, a saturated delta for uint64)."""
+    rng = np.random.default_rng(seed)
+    i = np.arange(n)
+    per = rng.integers(300, 1500, n // 300 + 1)
+    starts = np.concatenate([[0], np.cumsum(per)])
+    seg = np.searchsorted(starts, i, side="right") - 1
+    k = i - starts[seg]
+    h = rng.integers(500, 5000, seg.max() + 1)
+    slope = rng.integers(5, 40, seg.max() + 1)
+    return (np.maximum(0, h[seg] - slope[seg] * k) + rng.integers(0, 3, n)).astype(dtype)
+
+
+@pytest.mark.parametrize("dt", [np.uint32, np.uint64])
+def test_splitmerge_predictions_below_zero(gpu, dt):
+    # compression/CpuPartitioner.cuh:106 static_cast<T>(std::round(pred)) on
+    # predictions below 0: the library's host partitioner and the GPU
+    # split/merge both follow the x86 conversion of the reference's host build
+    v = hockey_sticks(1 << 16, dt, 3)
+    full_case(v, "cpu", 128)
+    st, en = glc.partition(to_dev(v), glc.PART_GPU_SPLITMERGE, 128)
+    ref = O.compress(v, O.PART_SPLITMERGE_EXACT, 128)
+    assert np.array_equal(st, ref.start) and np.array_equal(en, ref.end)
+
+
 def test_gpu_splitmerge_concurrent_calls(gpu):
     """The split/merge scratch is kept per device and shared by every call:
     calls from two threads at once (ctypes releases the GIL) on columns of
