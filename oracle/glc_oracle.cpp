@@ -235,9 +235,11 @@ inline int32_t fit_block_size(int32_t len0) {
 // form, decompression/FullDecompressionKernels.cuh:161).
 // fit_form 1 = no contraction in the four sums / det / num / theta0 (kept to
 // show where the contraction choice matters); the prediction stays fused.
-// Pinned by tests/test_oracle_fit.py: exact-rational normal equations where the
-// sums are exact (form-independent); the contraction choice itself is parity
-// unpinned (no NVIDIA toolchain here).
+// Pinned by tests/test_oracle_fit.py (exact-rational normal equations where the
+// sums are exact; the truncated tree of a non-power-of-two bs) and
+// tests/test_oracle_fit_exact.py (both forms derived one IEEE operation at a
+// time where the sums are inexact); which form nvcc used is parity unpinned
+// (no NVIDIA toolchain here).
 template <class T>
 void fit_partition(const T* v, int32_t s, int32_t e, int32_t bs, int fit_form, int32_t& type, double* prm,
                    int32_t& bits, int64_t& err) {
@@ -305,7 +307,7 @@ void fit_partition(const T* v, int32_t s, int32_t e, int32_t bs, int fit_form, i
 }
 
 // compression/CompressionKernels.cuh:222-238: exclusive prefix of len*bits.
-// Pinned by tests/test_oracle_pack.py (closed form).
+// Pinned by tests/test_oracle_bits.py (closed form).
 inline int64_t bit_offsets(Col& c) {
     const size_t P = c.start.size();
     c.off.assign(P, 0);
@@ -341,7 +343,7 @@ inline bool well_formed(const int32_t* st, const int32_t* en, int64_t P, int64_t
 // compression/CompressionKernels.cuh:243-372 (packDeltasKernelOptimized),
 // restated per partition; identical output because, for well-formed
 // partitions, the per-element search of the reference finds exactly the
-// partition iterated here.  Pinned by tests/test_oracle_pack.py (independent
+// partition iterated here.  Pinned by tests/test_oracle_bits.py (independent
 // bit-by-bit packer) and by round trips.
 template <class T> void pack(const T* v, Col& c) {
     const int64_t P = static_cast<int64_t>(c.start.size());
@@ -511,8 +513,9 @@ inline double warp_tree_sum(const double* lane) {
 // Sum2_w/n - (Sum_w/n)^2 (contracted to fma(-q,q,Sum2_w/n)) converted to float
 // and accumulated by float atomicAdd (:689-691) -- here in warp order 0..3,
 // one of the orders the reference's atomics can produce.
-// Pinned by tests/test_oracle_v6.py (closed form on constant blocks, float64
-// re-evaluation within tolerance).
+// Pinned by tests/test_oracle_partitioners.py (closed form on constant blocks,
+// float64 re-evaluation) and tests/test_oracle_fit_exact.py (the float bits
+// derived one IEEE operation at a time in this order).
 template <class T> float v6_block_stat(const T* v, int64_t s, int64_t e) {
     const int64_t n = e - s;
     double sum[128], ssq[128];
@@ -579,8 +582,10 @@ void v6_partitions(const T* v, int64_t n, int32_t base, std::vector<int32_t>& st
 // compiled by the reference's Makefile without -march (no FMA): plain
 // double arithmetic, sequential sums.  Model bytes = sizeof(PartitionInfo) = 80
 // (:18, core/InternalTypes.cuh:8-17).
-// Pinned by tests/test_oracle_cpu_partitioner.py (known answers on linear and
-// step data, invariants); greedy ties beyond those are parity unpinned.
+// Pinned by tests/test_oracle_partitioners.py (known answers on linear and
+// step data, invariants), tests/test_oracle_x86_cast.py (costs where the fit
+// leaves T's range) and tests/test_oracle_splitmerge.py (costs against big
+// integer sums); greedy ties beyond those are parity unpinned.
 template <class T> struct CpuPartitioner {
     const T* v;
     int64_t n;
