@@ -101,39 +101,129 @@ V6Scratch& v6_scratch(int device) {
     return *e;
 }
 
+// Device blocks of freed columns (metadata arenas and payloads), kept per
+// device for the next compress or deserialize: a hipMalloc/hipFree pair of a
+// few hundred MB costs 0.1-0.3 ms, comparable to the compress kernels of a
+// 2 GiB column.  A block is put back only after hipDeviceSynchronize, the
+// same wait hipFree makes, so no kernel still reading the freed column can
+// see the block's next owner.  At most GLC_ALLOC_CACHE_MIB MiB (default 4096)
+// stay cached per device; when hipMalloc fails the device's cache is
+// released and the allocation tried again.
+struct BlockCache {
+    std::mutex mu;
+    std::multimap<int64_t, void*> blocks;  // bytes -> block
+    int64_t held = 0;
+};
+
+BlockCache& block_cache(int device) {
+    static std::mutex table_mu;
+    static std::map<int, BlockCache*>* table = new std::map<int, BlockCache*>();  // used until exit
+    std::lock_guard<std::mutex> g(table_mu);
+    BlockCache*& e = (*table)[device];
+    if (!e) e = new BlockCache();
+    return *e;
+}
+
+int64_t block_cache_cap() {
+    static const int64_t cap = [] {
+        const char* e = std::getenv("GLC_ALLOC_CACHE_MIB");
+        const long long mib = e ? std::atoll(e) : 4096;
+        return static_cast<int64_t>(mib < 0 ? 0 : mib) << 20;
+    }();
+    return cap;
+}
+
+// A device block of at least `bytes`: a cached one no larger than twice the
+// request (plus 1 MiB), else a new one.  *got receives the block's size.
+void* block_get(int device, int64_t bytes, int64_t* got) {
+    bytes = std::max<int64_t>(bytes, 256);
+    {
+        BlockCache& bc = block_cache(device);
+        std::lock_guard<std::mutex> g(bc.mu);
+        auto it = bc.blocks.lower_bound(bytes);
+        if (it != bc.blocks.end() && it->first <= 2 * bytes + (1 << 20)) {
+            void* p = it->second;
+            *got = it->first;
+            bc.held -= it->first;
+            bc.blocks.erase(it);
+            return p;
+        }
+    }
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        BlockCache& bc = block_cache(device);
+        {
+            std::lock_guard<std::mutex> g(bc.mu);
+            for (auto& kv : bc.blocks) (void)hipFree(kv.second);
+            bc.blocks.clear();
+            bc.held = 0;
+        }
+        p = nullptr;
+        if (hipMalloc(&p, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+    }
+    *got = bytes;
+    return p;
+}
+
+void block_put(int device, void* p, int64_t bytes) {
+    if (!p) return;
+    BlockCache& bc = block_cache(device);
+    if (bytes > 0 && bytes <= block_cache_cap() && hipDeviceSynchronize() == hipSuccess) {
+        std::lock_guard<std::mutex> g(bc.mu);
+        // drop the largest blocks until this one fits under the cap
+        while (bc.held + bytes > block_cache_cap() && !bc.blocks.empty()) {
+            auto last = std::prev(bc.blocks.end());
+            (void)hipFree(last->second);
+            bc.held -= last->first;
+            bc.blocks.erase(last);
+        }
+        bc.blocks.emplace(bytes, p);
+        bc.held += bytes;
+        return;
+    }
+    (void)hipGetLastError();
+    (void)hipFree(p);
+}
+
 int alloc_column(glc_column* c, int64_t P, hipStream_t s) {
     c->P = P;
     c->n_index = (c->n + kIndexTile - 1) / kIndexTile + 1;
     ArenaLayout L = layout_for(P, c->n_index);
-    void* arena = nullptr;
-    if (hipMalloc(&arena, L.total) != hipSuccess) {
-        (void)hipGetLastError();
-        return fail(GLC_ERR_OOM, "device allocation of column metadata failed (" + std::to_string(L.total) + " B)");
-    }
+    void* arena = block_get(c->device, L.total, &c->meta_block);
+    if (!arena) return fail(GLC_ERR_OOM, "device allocation of column metadata failed (" + std::to_string(L.total) + " B)");
     bind_arena(c, static_cast<char*>(arena), L);
     GLC_HIP(hipMemsetAsync(arena, 0, L.total, s));
     return GLC_OK;
 }
 
-
-int alloc_payload(glc_column* c, int64_t payload_words, hipStream_t s) {
+// zero = false: the caller writes or zeroes every payload word itself
+// (compress: the pack kernels plus launch_zero_edges).
+int alloc_payload(glc_column* c, int64_t payload_words, hipStream_t s, bool zero = true) {
     c->payload_words = payload_words;
     c->words_alloc = payload_words + kPayloadPadWords;
-    void* p = nullptr;
-    if (hipMalloc(&p, c->words_alloc * 4) != hipSuccess) {
-        (void)hipGetLastError();
+    void* p = block_get(c->device, c->words_alloc * 4, &c->words_block);
+    if (!p)
         return fail(GLC_ERR_OOM, "device allocation of the payload failed (" + std::to_string(c->words_alloc * 4) +
                                      " B)");
-    }
     c->d_words = static_cast<uint32_t*>(p);
-    GLC_HIP(hipMemsetAsync(p, 0, c->words_alloc * 4, s));
+    if (zero) GLC_HIP(hipMemsetAsync(p, 0, c->words_alloc * 4, s));
     return GLC_OK;
+}
+
+void release_words(glc_column* c) {
+    block_put(c->device, c->d_words, c->words_block);
+    c->d_words = nullptr;
+    c->words_block = 0;
 }
 
 void destroy(glc_column* c) {
     if (!c || c->view) return;
-    if (c->meta_arena) (void)hipFree(c->meta_arena);
-    if (c->d_words) (void)hipFree(c->d_words);
+    if (c->meta_arena) block_put(c->device, c->meta_arena, c->meta_block);
+    if (c->d_words) release_words(c);
     if (c->d_plain) (void)hipFree(c->d_plain);
     delete c;
 }
@@ -307,7 +397,8 @@ int compress_impl(int dtype, const void* d_values, int64_t n, int partitioner, i
     if ((rc = finish_index(c, s, &sum))) return bail(rc);
     c->total_bits = sum.total_bits;
     c->fixed_size = sum.not_fixed ? 0 : static_cast<int32_t>(len0);
-    if ((rc = alloc_payload(c, (c->total_bits + 31) / 32, s))) return bail(rc);
+    if ((rc = alloc_payload(c, (c->total_bits + 31) / 32, s, false))) return bail(rc);
+    if ((rc = launch_zero_edges(c, s))) return bail(rc);
     if ((rc = launch_pack(dtype, d_values, c, s))) return bail(rc);
     GLC_HIP_BAIL(hipStreamSynchronize(s));
     *out = c;
@@ -515,7 +606,7 @@ int glc_column_info_get(const glc_column* c, glc_column_info* info) {
     info->has_packed = c->d_words != nullptr;
     info->has_preunpacked = c->d_plain != nullptr;
     info->fit_block_size = c->fit_block;
-    info->device_bytes = c->meta_bytes + (c->d_words ? c->words_alloc * 4 : 0) + (c->d_plain ? c->n * 8 : 0);
+    info->device_bytes = c->meta_block + (c->d_words ? c->words_block : 0) + (c->d_plain ? c->n * 8 : 0);
     return GLC_OK;
 }
 
@@ -635,8 +726,7 @@ int glc_preunpack(glc_column* c, int free_packed, void* stream) {
     if (c->d_plain) {
         if (free_packed && c->d_words) {
             GLC_HIP(hipStreamSynchronize(s));
-            (void)hipFree(c->d_words);
-            c->d_words = nullptr;
+            release_words(c);
         }
         return GLC_OK;
     }
@@ -655,8 +745,7 @@ int glc_preunpack(glc_column* c, int free_packed, void* stream) {
     c->d_plain = static_cast<int64_t*>(p);
     if (free_packed) {  // G:2284-2285
         GLC_HIP(hipStreamSynchronize(s));
-        (void)hipFree(c->d_words);
-        c->d_words = nullptr;
+        release_words(c);
     }
     return GLC_OK;
 }

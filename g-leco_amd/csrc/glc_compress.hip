@@ -1205,6 +1205,35 @@ int launch_build_tinfo(glc_column* c, hipStream_t s) {
     return GLC_OK;
 }
 
+// Single-column form of k_zero_edges_batch: tile t's first payload word and
+// the three words ending at the one past its last element's first bit are
+// the only words the pack kernels write with atomicOr (merge_edges' outer
+// edges, pack_tile's k == 0 / k >= nw - 3); every other word of the payload
+// is stored whole, so these are all a compress must clear.
+__global__ void k_zero_edges(const Desc* __restrict__ desc, const int32_t* __restrict__ tile_part, int64_t n,
+                             int64_t ntiles, uint32_t* __restrict__ words) {
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= ntiles) return;
+    const int64_t e0 = t * kIndexTile;
+    const int64_t e1 = (e0 + kIndexTile < n) ? e0 + kIndexTile : n;
+    const int64_t plo = tile_part[e0 >> kIndexShift];
+    const int64_t phi = tile_part[(e1 + kIndexTile - 1) >> kIndexShift];
+    const int64_t wbase = bitpos_lb(desc, plo, phi, e0) >> 5;
+    const int64_t last = bitpos_lb(desc, plo, phi, e1 - 1);
+    const int64_t wlast = (last >= 0) ? (last + 64) >> 5 : wbase;
+    words[wbase] = 0u;
+    for (int64_t w = (wlast - 2 > wbase) ? wlast - 2 : wbase + 1; w <= wlast; ++w) words[w] = 0u;
+}
+
+int launch_zero_edges(glc_column* c, hipStream_t s) {
+    const int64_t ntiles = c->n_index - 1;
+    if (c->P == 0 || c->n == 0 || c->total_bits == 0 || ntiles <= 0) return GLC_OK;
+    hipLaunchKernelGGL(k_zero_edges, dim3(grid_for(ntiles, 256)), dim3(256), 0, s, c->d_desc, c->d_tile_part, c->n,
+                       ntiles, c->d_words);
+    GLC_HIP(hipGetLastError());
+    return GLC_OK;
+}
+
 int launch_pack(int dtype, const void* d_values, glc_column* c, hipStream_t s) {
     if (c->P == 0 || c->n == 0 || c->total_bits == 0) return GLC_OK;
     return dispatch_dtype(dtype, [&](auto t) {

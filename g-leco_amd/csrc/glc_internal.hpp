@@ -193,6 +193,8 @@ struct glc_column {
     int32_t* d_sched = nullptr;     // work-stealing queues (zeroed; reset by the kernel that uses them)
     uint32_t* d_words = nullptr;    // payload (nullptr after preunpack with free_packed)
     int64_t words_alloc = 0;
+    int64_t meta_block = 0;         // bytes of the device blocks behind meta_arena / d_words
+    int64_t words_block = 0;        // (>= what the column uses when a cached block was reused)
     int64_t* d_plain = nullptr;     // pre-unpacked deltas
     bool view = false;              // arrays belong to a batch; destroy frees nothing
 };
@@ -264,6 +266,9 @@ template <class T> __host__ __device__ inline T x86_cast(double r) {
 int launch_fit(int dtype, const void* d_values, glc_column* c, int32_t bs, int fit_form, hipStream_t s);
 int launch_scan_offsets(glc_column* c, int64_t* d_total, hipStream_t s);
 int launch_pack(int dtype, const void* d_values, glc_column* c, hipStream_t s);
+// zero the payload words a compress leaves partly written by atomics (each
+// 1024-value tile's first word and last three words), after the index
+int launch_zero_edges(glc_column* c, hipStream_t s);
 int launch_build_index(glc_column* c, IndexSummary* d_sum, hipStream_t s);
 int launch_build_tinfo(glc_column* c, hipStream_t s);  // after a valid index
 // V6 boundaries into d_start/d_end[cap]; *d_total receives P, d_base[nvb]

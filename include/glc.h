@@ -110,7 +110,11 @@ int glc_compress(int dtype, const void* d_values, int64_t n, const glc_opts* opt
  * non-overlapping: the reference's compress() after its partition step. */
 int glc_compress_partitions(int dtype, const void* d_values, int64_t n, const int32_t* h_start,
                             const int32_t* h_end, int64_t num_partitions, void* stream, glc_column** out);
-/* G:793-821 cleanup(). NULL is a no-op. */
+/* G:793-821 cleanup(). NULL is a no-op.  Like cudaFree in the reference it
+ * waits for the device (hipDeviceSynchronize); the column's metadata and
+ * payload blocks then go to a per-device cache that later compress and
+ * deserialize calls reuse, at most GLC_ALLOC_CACHE_MIB MiB (default 4096) per
+ * device, released when an allocation would otherwise fail. */
 int glc_column_free(glc_column* col);
 int glc_column_info_get(const glc_column* col, glc_column_info* info);
 /* Copy the SoA metadata to host arrays (any may be NULL). params is [4*P]. */

@@ -72,3 +72,20 @@ def test_c3_full_size(gpu):
     col.decompress_full(glc.FULL_PREUNPACKED, out=out)
     torch.cuda.synchronize()
     assert torch.equal(out, x)
+
+
+def test_gpu_splitmerge_large(gpu):
+    # the GPU split/merge (compression/CpuPartitioner.cuh:141-242 cost model,
+    # exact 128-bit sums) on a 2^26-value piecewise-linear uint64 column with
+    # minimum size 128 (524,288 candidate starts): whole column against the
+    # oracle's exact-sum split/merge, boundaries, parameters and payload
+    print("split/merge: generate 2^26 uint64 on the GPU", flush=True)
+    x = W.c2_uint64_torch(1 << 26)
+    col = glc.compress(x, glc.PART_GPU_SPLITMERGE, 128)
+    torch.cuda.synchronize()
+    v = x.cpu().numpy()
+    print("split/merge: oracle", flush=True)
+    ref = O.compress(v, O.PART_SPLITMERGE_EXACT, 128)
+    compare(col, ref)
+    del ref, v
+    decoders(col, x, q=1 << 20)
