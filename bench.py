@@ -30,6 +30,18 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
 HBM_PEAK_GBS = 8000.0
 META_BYTES_PER_PARTITION = 36
+# Cache-line bytes one random lookup must move: its 4-B position and 8-B
+# result (coalesced), one 64-B sector of the 32-B tile record and one 64-B
+# sector of the payload word(s) it decodes.
+LOOKUP_LINE_BYTES = 4 + 8 + 64 + 64
+# bench --mode -> the decode kernel that mode launches
+DECODER_KERNEL = {
+    "worksteal": "k_decode_wave (GLC_FULL_WORKSTEAL: one wave per 1024-value tile, tiles handed to CUs by the "
+                 "hardware workgroup dispatcher)",
+    "bitpacked": "k_decode_wave (GLC_FULL_BITPACKED)",
+    "v2": "k_decode_parts<persistent=false> (GLC_FULL_V2: one workgroup per partition)",
+    "worksteal_adv": "k_decode_parts<persistent=true> (GLC_FULL_WORKSTEAL_ADV: workgroups claim partitions from a counter)",
+}
 
 
 def log(msg):
@@ -242,6 +254,52 @@ def bench_c1(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_i32(args):
+    """int32 full decompress at 2^28 values: v_i = 2^21 + 3 i + U[-2^12, 2^12]
+    (13-14-bit residuals), fixed 4096-value partitions -- the 32-bit element
+    case of the decode kernel (4 bytes of output per value)."""
+    import torch
+
+    import glc
+
+    n = 1 << args.log2n
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev)
+    g.manual_seed(4242)
+    log(f"generating int32 column, N=2^{args.log2n}")
+    x = ((torch.arange(n, device=dev, dtype=torch.int64) * 3 + (1 << 21)) +
+         torch.randint(-(1 << 12), (1 << 12) + 1, (n,), generator=g, device=dev)).to(torch.int32)
+    col = glc.compress(x, glc.PART_FIXED, 4096)
+    torch.cuda.synchronize()
+    inf = col.info()
+    out = torch.empty_like(x)
+    for _ in range(args.warmup):
+        col.decompress_full(glc.FULL_BITPACKED, out=out)
+    torch.cuda.synchronize()
+    assert torch.equal(out, x), "int32 decode mismatch"
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    for a, c in ev:
+        a.record()
+        col.decompress_full(glc.FULL_BITPACKED, out=out)
+        c.record()
+    torch.cuda.synchronize()
+    ms = sum(a.elapsed_time(c) for a, c in ev) / args.steps
+    alg = (inf.total_bits + 7) // 8 + META_BYTES_PER_PARTITION * inf.num_partitions + 4 * n
+    line = {
+        "metric": "int32 full-decompress GB/s (% HBM roofline)",
+        "value": round(4 * n / (ms * 1e-3) / 1e9, 2), "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "int32", "data": "synthetic (seeded, generated on the GPU)",
+        "config": {"workload": f"int32_linear_noise13_N2^{args.log2n}_fixed4096", "n": n,
+                   "partitions": inf.num_partitions, "compressed_bytes": inf.compressed_size},
+        "roofline": {"bound": "hbm", "achieved": round(alg / (ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_decode_wave (int32, bit-packed)", "algorithmic_bytes": alg},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def bench_c3(args):
     """Sorted int64 timestamp-like column, N = 2^29, fixed 4096-value
     partitions: packed against pre-unpacked full decompress and random access
@@ -377,97 +435,89 @@ def bench_c5(args):
     nlook = sum(r[2].size for r in reqs if r[0] == "look")
     log(f"{len(reqs)} requests: {sum(r[0] == 'dec' for r in reqs)} decompress, {nlook} lookups")
 
-    def plain(lam, reqs):
-        arr = np.cumsum(rng.exponential(1.0 / lam, len(reqs))) if lam else np.zeros(len(reqs))
-        lat = []
-        t0 = time.perf_counter()
-        for k, (kind, c, pos) in enumerate(reqs):
-            ta = t0 + arr[k]
-            while time.perf_counter() < ta:
-                pass
-            col = cols[c]
-            if kind == "dec":
-                col.decompress_full(glc.FULL_AUTO)
-            else:
-                col.random_access(torch.from_numpy(pos).cuda())
-            torch.cuda.synchronize()
-            lat.append(time.perf_counter() - ta)
-        wall = time.perf_counter() - t0
-        lat = sorted(lat)
-        return {"p50_ms": 1e3 * lat[len(lat) // 2], "p99_ms": 1e3 * lat[int(0.99 * (len(lat) - 1))],
-                "req_per_s": len(reqs) / wall, "offered_req_per_s": lam}
-
+    # both paths are driven natively through the C-ABI (g-leco_amd/lib/libglc_replay.so):
+    # no interpreter work per request on either side
+    import ctypes as ct
+    R = ct.CDLL(os.path.join(ROOT, "g-leco_amd", "lib", "libglc_replay.so"))
+    vp, i64 = ct.c_void_p, ct.c_int64
+    R.glc_replay.restype = ct.c_int
+    R.glc_replay.argtypes = [ct.c_int, vp, vp, vp, i64, vp, vp, vp, vp, vp, vp, vp, ct.c_int, vp, i64, vp, vp, vp]
+    kinds = np.array([1 if r[0] == "dec" else 0 for r in reqs], np.int32)
+    rcol = np.array([r[1] for r in reqs], np.int32)
+    cnt = np.array([0 if r[0] == "dec" else r[2].size for r in reqs], np.int64)
+    off = np.concatenate([[0], np.cumsum(cnt)[:-1]]).astype(np.int64)
+    h_pos = np.concatenate([r[2] for r in reqs if r[0] == "look"]).astype(np.int32)
+    d_pos = torch.from_numpy(h_pos).to("cuda")
+    out_bytes = int(max(lens.max(), cnt.max())) * 8
+    d_out = torch.empty(out_bytes, dtype=torch.uint8, device="cuda")
+    col_ptrs = (ct.c_void_p * C)(*[c.handle.value for c in cols])
     budget = int(args.c5_budget_gib * 2 ** 30)
     srv = glc.Server(budget, max_inflight=args.c5_max_inflight)
-    ids = [srv.add_column(c) for c in cols]
+    ids = np.array([srv.add_column(c) for c in cols], np.int32)
+    P_ = lambda a: a.ctypes.data_as(ct.c_void_p)
 
-    def served(lam, reqs):
-        arr = np.cumsum(rng.exponential(1.0 / lam, len(reqs)))
-        rid = [None] * len(reqs)
-        srv.reset_stats()
-        t0 = time.perf_counter()
-
-        nth = max(1, args.c5_clients)
-
-        def client(t):  # submitting thread t takes requests t, t + nth, ... at their arrival times
-            for k in range(t, len(reqs), nth):
-                kind, c, pos = reqs[k]
-                ta = t0 + arr[k]
-                while time.perf_counter() < ta:
-                    time.sleep(0)
-                rid[k] = srv.submit_decompress(ids[c]) if kind == "dec" else srv.submit_lookup(ids[c], pos)
-
-        ths = [threading.Thread(target=client, args=(t,)) for t in range(nth)]
-        for th in ths:
-            th.start()
-        for k in range(len(reqs)):
-            while rid[k] is None:
-                time.sleep(20e-6)
-            srv.wait(rid[k], 120000)
-            srv.release(rid[k])
-        for th in ths:
-            th.join()
-        wall = time.perf_counter() - t0
-        st = srv.stats()
-        return {"p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "p99_lookup_ms": st["p99_lookup_ms"],
-                "p99_decompress_ms": st["p99_decompress_ms"], "req_per_s": len(reqs) / wall,
-                "offered_req_per_s": lam, "mean_requests_per_step": st["mean_requests_per_step"],
-                "inflight_peak": st["inflight_peak"], "pool_peak_gib": st["pool_peak_bytes"] / 2**30}
+    def replay(mode, lam, nth=1):
+        nreq = len(reqs)
+        arr = np.cumsum(rng.exponential(1.0 / lam, nreq)) if lam else np.zeros(nreq)
+        arr = np.ascontiguousarray(arr, np.float64)
+        lat = np.zeros(nreq, np.float64)
+        wall, rps = ct.c_double(), ct.c_double()
+        torch.cuda.synchronize()
+        rc = R.glc_replay(mode, col_ptrs, srv._h if mode >= 1 else None, P_(ids), nreq, P_(kinds), P_(rcol),
+                          P_(off), P_(cnt), P_(h_pos), ct.c_void_p(d_pos.data_ptr()), P_(arr), nth,
+                          ct.c_void_p(d_out.data_ptr()), out_bytes, P_(lat), ct.byref(wall), ct.byref(rps))
+        assert rc == 0, f"replay failed: {rc} {glc.lib().glc_last_error()}"
+        s_ = np.sort(lat)
+        row = {"p50_ms": float(s_[nreq // 2]), "p99_ms": float(s_[int(0.99 * (nreq - 1))]),
+               "req_per_s": nreq / wall.value, "offered_req_per_s": lam}
+        if mode >= 1:
+            st = srv.stats()
+            row.update({"mean_requests_per_step": rps.value, "steps": st["steps"],
+                        "p99_lookup_ms": st["p99_lookup_ms"], "p99_decompress_ms": st["p99_decompress_ms"],
+                        "inflight_peak": st["inflight_peak"], "pool_peak_gib": st["pool_peak_bytes"] / 2**30})
+        return row
 
     # equality spot check: a few results through both paths
     for k in range(0, len(reqs), max(1, len(reqs) // 8)):
         kind, c, pos = reqs[k]
         a = (cols[c].decompress_full() if kind == "dec" else cols[c].random_access(torch.from_numpy(pos).cuda()))
-        r = srv.submit_decompress(ids[c]) if kind == "dec" else srv.submit_lookup(ids[c], pos)
+        r = srv.submit_decompress(int(ids[c])) if kind == "dec" else srv.submit_lookup(int(ids[c]), pos)
         srv.wait(r)
-        assert np.array_equal(srv.result(r, ids[c]), a.cpu().numpy()), "served result differs"
+        assert np.array_equal(srv.result(r, int(ids[c])), a.cpu().numpy()), "served result differs"
         srv.release(r)
     log("served results equal the plain path; warming up")
-    plain(0, reqs[:20])
-    served(1e5, reqs[:50])
+    replay(0, 0)
+    replay(2, 1e6, args.c5_clients)
 
-    cap = plain(0, reqs[: args.c5_probe])  # back-to-back: the plain path's capacity
+    cap = replay(0, 0)  # back-to-back: the plain path's capacity
     mu = cap["req_per_s"]
-    log(f"plain capacity {mu:.1f} req/s (p50 {cap['p50_ms']:.2f} ms)")
+    log(f"plain capacity {mu:.1f} req/s")
     rows = {"plain_closed_loop": cap}
     for f in (0.5, 0.8):
-        rows[f"plain_{f}mu"] = plain(f * mu, reqs)
+        rows[f"plain_{f}mu"] = replay(0, f * mu)
         log(f"plain at {f}mu: {rows[f'plain_{f}mu']}")
-    for f in (0.5, 0.8, 1.5, 3.0, 6.0):
-        rows[f"served_{f}mu"] = served(f * mu, reqs)
+    # served rows with resident device positions (what the plain path gets),
+    # then with host positions (the server copies them at submit)
+    for f in (0.5, 0.8, 1.0, 1.25, 1.5, 2.0, 3.0, 6.0):
+        rows[f"served_{f}mu"] = replay(2, f * mu, args.c5_clients)
         log(f"served at {f}mu: {rows[f'served_{f}mu']}")
+    for f in (0.8, 1.5, 3.0):
+        rows[f"served_hostpos_{f}mu"] = replay(1, f * mu, args.c5_clients)
+        log(f"served (host positions) at {f}mu: {rows[f'served_hostpos_{f}mu']}")
     ref_p99 = rows["plain_0.8mu"]["p99_ms"]
-    ok = [v for k, v in rows.items() if k.startswith("served") and v["p99_ms"] <= ref_p99]
+    ok = [v for k, v in rows.items() if k.startswith("served_") and "hostpos" not in k and v["p99_ms"] <= ref_p99]
     best = max(ok, key=lambda v: v["req_per_s"]) if ok else None
     line = {
         "metric": "serving: requests/s at p99 latency no worse than the plain path's at 0.8 of its capacity",
         "value": round(best["req_per_s"], 1) if best else None, "unit": "req/s", "n_gpus": 1,
-        "steps": len(reqs), "warmup": 50, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
+        "steps": len(reqs), "warmup": 2, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "u64/i64", "data": "synthetic (seeded, generated on the GPU)",
         "config": {"workload": f"c5_serving_{C}_columns_2^{lo}..2^{hi}", "columns": C, "values": int(lens.sum()),
                    "compressed_gib": round(comp / 2**30, 3), "requests": len(reqs), "lookups": int(nlook),
                    "budget_gib": args.c5_budget_gib, "max_inflight": args.c5_max_inflight,
-                   "served_client_threads": args.c5_clients},
+                   "clients": "native (libglc_replay): plain = 1 thread, one request at a time; served = "
+                              f"{args.c5_clients} submitting threads; positions resident on the device for both "
+                              "(served_hostpos rows: host positions copied at submit)"},
         "plain_capacity_req_per_s": round(mu, 1),
         "speedup_vs_plain_at_equal_p99": round(best["req_per_s"] / rows["plain_0.8mu"]["req_per_s"], 3) if best else None,
         "rows": {k: {kk: (round(vv, 3) if isinstance(vv, float) else vv) for kk, vv in v.items()} for k, v in rows.items()},
@@ -695,7 +745,7 @@ def main():
     ap.add_argument("--mode", default="worksteal", choices=["worksteal", "bitpacked", "v2", "worksteal_adv"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-log2", type=int, default=24)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "e2e", "splitmerge"])
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "e2e", "splitmerge", "i32"])
     ap.add_argument("--c5-columns", type=int, default=512)
     ap.add_argument("--c5-min-log2", type=int, default=20)
     ap.add_argument("--c5-max-log2", type=int, default=26)
@@ -703,7 +753,7 @@ def main():
     ap.add_argument("--c5-probe", type=int, default=200)
     ap.add_argument("--c5-budget-gib", type=float, default=16.0)
     ap.add_argument("--c5-max-inflight", type=int, default=256)
-    ap.add_argument("--c5-clients", type=int, default=2, help="threads submitting the served rows' requests")
+    ap.add_argument("--c5-clients", type=int, default=4, help="native threads submitting the served rows' requests")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain device copy of the input (known bytes) for counter calibration")
     ap.add_argument("--c4-columns", type=int, default=10_000)
@@ -723,6 +773,8 @@ def main():
         return bench_c5(args)
     if args.workload == "c1":
         return bench_c1(args)
+    if args.workload == "i32":
+        return bench_i32(args)
     if args.workload == "e2e":
         return bench_e2e(args)
     if args.workload == "splitmerge":
@@ -807,13 +859,48 @@ def main():
         torch.cuda.synchronize()
         del y
 
-    traffic = None
+    # per-request lookup latency: 1000 requests of 2^10 resident positions,
+    # one at a time (launch, then wait), host wall clock per request; plus the
+    # device time of each request's kernel (HIP events on the launch stream)
+    rq, nreq = 1 << 10, 1000
+    rpos = torch.from_numpy(W.positions_mixed(rq * nreq, n, seed=4242)).to(dev)
+    rout = torch.empty(rq, dtype=torch.uint64, device=dev)
+    for k in range(20):
+        col.random_access(rpos[k * rq:(k + 1) * rq], out=rout)
+    torch.cuda.synchronize()
+    req_wall, req_dev = [], []
+    rev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(nreq)]
+    for k in range(nreq):
+        a, b = rev[k]
+        t = time.perf_counter()
+        a.record()
+        col.random_access(rpos[k * rq:(k + 1) * rq], out=rout)
+        b.record()
+        b.synchronize()
+        req_wall.append((time.perf_counter() - t) * 1e3)
+    req_dev = sorted(a.elapsed_time(b) for a, b in rev)
+    req_wall.sort()
+    pq = lambda a, f: a[min(len(a) - 1, int(round(f * (len(a) - 1))))]
+
+    # FETCH_SIZE / WRITE_SIZE per decode launch, measured by two rocprofv3
+    # --pmc runs of this program (scaled by the same counters on a plain copy)
+    # and kept with the commit they were measured on
+    traffic, traffic_src = None, None
     tf = os.path.join(ROOT, "profiles", "decode_traffic.json")
     if os.path.exists(tf):
         try:
-            traffic = json.load(open(tf)).get("bytes_per_launch")
+            tj = json.load(open(tf))
+            traffic = tj.get("bytes_per_launch")
+            traffic_src = f"profiles/decode_traffic.json (rocprofv3 --pmc, measured at commit {tj.get('commit', '?')})"
         except Exception:
             traffic = None
+    look_traffic = None
+    lf = os.path.join(ROOT, "profiles", "lookup_traffic.json")
+    if os.path.exists(lf):
+        try:
+            look_traffic = json.load(open(lf)).get("fetch_bytes_per_lookup")
+        except Exception:
+            look_traffic = None
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -834,7 +921,7 @@ def main():
         "dtype": "u64",
         "data": "synthetic (seeded generator of the configured shape, generated on the GPU)",
         "config": {"workload": f"c2_uint64_piecewise_linear_N2^{args.log2n}_v6", "n": n, "partitioner": "v6",
-                   "v6_base": 2048, "decoder": args.mode, "lookups_per_step": q,
+                   "v6_base": 2048, "decoder": DECODER_KERNEL[args.mode], "lookups_per_step": q,
                    "lookup_mix": "1/2 uniform + 1/2 zipf(1.1) scrambled", "partitions": inf.num_partitions,
                    "compressed_bytes": inf.compressed_size, "compression_ratio": round(n * 8 / inf.compressed_size, 4)},
         "decode_ms_avg": round(dec_avg, 4),
@@ -842,9 +929,18 @@ def main():
         "lookup_mlookups_per_s": round(q / (look_avg * 1e-3) / 1e6, 1),
         "lookup_batch_ms_p50": round(p(0.5), 4),
         "lookup_batch_ms_p99": round(p(0.99), 4),
+        "lookup_request": {"positions": rq, "requests": nreq, "resident_positions": True,
+                           "wall_ms_p50": round(pq(req_wall, 0.5), 4), "wall_ms_p99": round(pq(req_wall, 0.99), 4),
+                           "device_ms_p50": round(pq(req_dev, 0.5), 4), "device_ms_p99": round(pq(req_dev, 0.99), 4)},
+        "roofline_lookup": {"bound": "hbm", "kernel": "k_random_access<uint64>",
+                            "bytes_per_lookup": LOOKUP_LINE_BYTES,
+                            "achieved": round(q * LOOKUP_LINE_BYTES / (look_avg * 1e-3) / 1e9, 1),
+                            "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": round(q * LOOKUP_LINE_BYTES / (look_avg * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                            "traffic_bytes_per_lookup": look_traffic},
         "compress_ms": round(compress_ms_steady, 2), "compress_ms_first_call": round(compress_ms, 2),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_src,
                      "kernel": "k_decode_wave<uint64, packed>", "algorithmic_bytes": alg_bytes},
         "cpu_baseline": cpu,
     }

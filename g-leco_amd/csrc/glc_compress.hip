@@ -258,6 +258,130 @@ __device__ __forceinline__ void fit_wave(const T* __restrict__ x, int32_t n, int
     }
 }
 
+// Partitions of kFitRegMin..kFitRegMax values (block size 256): one
+// 256-thread workgroup per partition, i.e. the reference's block itself.
+// Thread t loads its values i = t, t+256, ... (at most kFitRegK of them) into
+// registers once, sums them in increasing i, and the tree s = 128, 64 runs
+// through LDS, s = 32..1 by shuffles in wave 0; the max-error pass then reads
+// the same registers, so the partition crosses HBM once.  Measured on int64
+// 2^29 with 4096-value partitions: 0.76 ms against 1.10 ms for the wave form.
+// Smaller partitions stay with the wave form (a workgroup per 1024 values
+// spends more on its barriers than the second read costs), and so do larger
+// ones (32 values per thread in registers cost occupancy).
+constexpr int kFitRegK = 16;
+constexpr int32_t kFitRegMin = 2048;
+constexpr int32_t kFitRegMax = 256 * kFitRegK;
+
+template <class T, int FORM>
+__global__ __launch_bounds__(256) void k_fit_reg(const T* __restrict__ v, const int32_t* __restrict__ st,
+                                                 const int32_t* __restrict__ en, int32_t* __restrict__ type,
+                                                 double* __restrict__ params, int32_t* __restrict__ bits,
+                                                 int64_t* __restrict__ err, int64_t P) {
+    __shared__ double s_sum[4][128];
+    __shared__ double s_th[2];
+    __shared__ int64_t s_mx[4];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int64_t p = blockIdx.x;
+    if (p >= P) return;
+    const int32_t s0 = st[p];
+    const int32_t n = en[p] - s0;
+    if (n < kFitRegMin || n > kFitRegMax) return;  // the wave form's partitions (uniform per workgroup)
+    const T* __restrict__ x = v + s0;
+    T y[kFitRegK];
+#pragma unroll
+    for (int k = 0; k < kFitRegK; ++k) {  // branch-free: every load in flight at once
+        const int32_t i = tid + 256 * k;
+        y[k] = x[i < n ? i : n - 1];
+    }
+    double a = 0.0, b = 0.0, c = 0.0, d = 0.0;
+    int ovf = 0;
+#pragma unroll
+    for (int k = 0; k < kFitRegK; ++k) {  // CompressionKernels.cuh:113-120, thread t's own order
+        const int32_t i = tid + 256 * k;
+        if (i < n) {
+            ovf |= might_overflow_double<T>(y[k]) ? 1 : 0;
+            const double xi = static_cast<double>(i);
+            const double yi = static_cast<double>(y[k]);
+            a = a + xi;
+            b = b + yi;
+            c = acc_prod<FORM>(xi, xi, c);
+            d = acc_prod<FORM>(xi, yi, d);
+        }
+    }
+    if (__syncthreads_or(ovf)) {  // :95-106 DIRECT_COPY
+        if (tid == 0) {
+            type[p] = MODEL_DIRECT_COPY;
+            params[4 * p] = 0.0; params[4 * p + 1] = 0.0; params[4 * p + 2] = 0.0; params[4 * p + 3] = 0.0;
+            err[p] = 0;
+            bits[p] = static_cast<int32_t>(sizeof(T) * 8);
+        }
+        return;
+    }
+    // tree (:129-137): s = 128 (t < 128 adds t + 128), s = 64, then 32..1 in wave 0
+    if (tid >= 128) {
+        s_sum[0][tid - 128] = a; s_sum[1][tid - 128] = b; s_sum[2][tid - 128] = c; s_sum[3][tid - 128] = d;
+    }
+    __syncthreads();
+    if (tid < 128) {
+        a = a + s_sum[0][tid]; b = b + s_sum[1][tid]; c = c + s_sum[2][tid]; d = d + s_sum[3][tid];
+    }
+    __syncthreads();
+    if (tid >= 64 && tid < 128) {
+        s_sum[0][tid - 64] = a; s_sum[1][tid - 64] = b; s_sum[2][tid - 64] = c; s_sum[3][tid - 64] = d;
+    }
+    __syncthreads();
+    if (wave == 0) {
+        a = a + s_sum[0][tid]; b = b + s_sum[1][tid]; c = c + s_sum[2][tid]; d = d + s_sum[3][tid];
+        for (int h = 32; h > 0; h >>= 1) {
+            a = a + __shfl_down(a, h, 64);
+            b = b + __shfl_down(b, h, 64);
+            c = c + __shfl_down(c, h, 64);
+            d = d + __shfl_down(d, h, 64);
+        }
+        if (lane == 0) {
+            double t0, t1;
+            fit_theta<FORM>(n, a, b, c, d, t0, t1);
+            s_th[0] = t0;
+            s_th[1] = t1;
+        }
+    }
+    __syncthreads();
+    const double t0 = s_th[0], t1 = s_th[1];
+    int64_t mx = 0;
+    // the clamped index repeats element n - 1: same value
+    if (enc_fast_range<T>(__builtin_fma(t1, 0.0, t0), __builtin_fma(t1, static_cast<double>(n - 1), t0))) {
+#pragma unroll
+        for (int k = 0; k < kFitRegK; ++k) {
+            const int32_t i = tid + 256 * k;
+            const int64_t dl = enc_delta_fast<T>(y[k], t0, t1, i < n ? i : n - 1);
+            const int64_t ab = dl < 0 ? static_cast<int64_t>(0ULL - static_cast<uint64_t>(dl)) : dl;
+            mx = ab > mx ? ab : mx;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < kFitRegK; ++k) {
+            const int32_t i = tid + 256 * k;
+            const int64_t ab = abs_delta<T>(y[k], t0, t1, i < n ? i : n - 1);
+            mx = ab > mx ? ab : mx;
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const int64_t q = __shfl_xor(mx, o, 64);
+        mx = q > mx ? q : mx;
+    }
+    if (lane == 0) s_mx[wave] = mx;
+    __syncthreads();
+    if (tid == 0) {
+        int64_t m = s_mx[0];
+        for (int w = 1; w < 4; ++w) m = s_mx[w] > m ? s_mx[w] : m;
+        type[p] = MODEL_LINEAR;
+        params[4 * p] = t0; params[4 * p + 1] = t1; params[4 * p + 2] = 0.0; params[4 * p + 3] = 0.0;
+        err[p] = m;
+        bits[p] = bits_for(m);
+    }
+}
+
 template <class T, int FORM>
 __global__ __launch_bounds__(64 * kFitWaves) void k_fit_wave(const T* __restrict__ v, const int32_t* __restrict__ st,
                                                             const int32_t* __restrict__ en, int32_t* __restrict__ type,
@@ -269,6 +393,7 @@ __global__ __launch_bounds__(64 * kFitWaves) void k_fit_wave(const T* __restrict
     const int32_t s0 = st[p];
     const int32_t n = en[p] - s0;
     if (n <= 0) return;  // :62, entry keeps its initial values
+    if (n >= kFitRegMin && n <= kFitRegMax) return;  // k_fit_reg's partitions
     fit_wave<T, FORM>(v + s0, n, p, type, params, bits, err, lane);
 }
 
@@ -1155,7 +1280,11 @@ inline unsigned grid_for(int64_t work, int64_t per_block, int64_t cap = 1LL << 3
 }  // namespace
 
 template <class T, int FORM> int launch_fit_t(const void* d_values, glc_column* c, int32_t bs, hipStream_t s) {
-    if (bs == kFitWaveBs) {
+    if (bs == kFitWaveBs) {  // two launches, each skipping the other's partitions
+        hipLaunchKernelGGL((k_fit_reg<T, FORM>), dim3(static_cast<unsigned>(c->P)), dim3(256), 0, s,
+                           static_cast<const T*>(d_values), c->d_start, c->d_end, c->d_type, c->d_params, c->d_bits,
+                           c->d_err, c->P);
+        GLC_HIP(hipGetLastError());
         const unsigned g = static_cast<unsigned>((c->P + kFitWaves - 1) / kFitWaves);
         hipLaunchKernelGGL((k_fit_wave<T, FORM>), dim3(g), dim3(64 * kFitWaves), 0, s, static_cast<const T*>(d_values),
                            c->d_start, c->d_end, c->d_type, c->d_params, c->d_bits, c->d_err, c->P);

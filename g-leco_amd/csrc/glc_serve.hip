@@ -29,6 +29,7 @@
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -116,6 +117,7 @@ struct Req {
     int64_t count = 0;            // lookups or values
     std::vector<int32_t> pos;     // lookup positions (host copy) when the arena had no room
     int64_t pos_off = -1;         // else: offset of the positions in the pinned position arena
+    const int32_t* dpos = nullptr;  // or: the caller's device positions, read in place
     int64_t out_off = -1;         // result block in the pool
     int64_t out_bytes = 0;
     int bypassed = 0;
@@ -161,6 +163,13 @@ struct glc_server {
     Slot slots[kLookupSlots];
     hipStream_t s_dec = nullptr;
     std::vector<Col> cols;
+
+    // The worker busy-polls (submissions, releases, events) while work is in
+    // flight and for spin_s seconds after its last activity, so neither a
+    // condition-variable wake-up nor a timed sleep sits between a request and
+    // its launch or its completion; after that it blocks on cv_work.
+    double spin_s = 0.002;
+    std::atomic<int64_t> work_seq{0};  // bumped by every submit and release
 
     std::mutex mu;
     std::condition_variable cv_work;  // worker: new submissions / releases / stop
@@ -273,14 +282,16 @@ int launch_step(glc_server* s, Slot& sl, std::vector<Req*>& look) {
         q.tinfo = c->d_tinfo;
         q.words = c->d_words;
         q.params = c->d_params;
-        q.pos = (r->pos_off >= 0) ? reinterpret_cast<const int32_t*>(s->parena_dev + r->pos_off) : dpos + in_slot;
+        q.pos = r->dpos ? r->dpos
+                        : (r->pos_off >= 0) ? reinterpret_cast<const int32_t*>(s->parena_dev + r->pos_off)
+                                            : dpos + in_slot;
         q.out = s->pool + r->out_off;
         q.n = c->n;
         q.first = first;
         q.dtype = c->dtype;
         q.fixed = 0;
         tab[i] = q;
-        if (r->pos_off < 0) {
+        if (!r->dpos && r->pos_off < 0) {
             std::memcpy(hpos + in_slot, r->pos.data(), 4 * r->count);
             in_slot += r->count;
         }
@@ -317,7 +328,7 @@ int launch_decompress(glc_server* s, Req* r) {
 }
 
 // Admit queued requests (arrival order, bounded bypass) and launch them.
-int admit_and_launch(glc_server* s) {
+int admit_and_launch(glc_server* s, int* launched) {
     std::vector<Req*> look, dec;
     Slot* slot = nullptr;
     {
@@ -337,7 +348,7 @@ int admit_and_launch(glc_server* s) {
         for (auto it = s->pending.begin(); it != s->pending.end();) {
             Req* r = *it;
             if (s->inflight >= s->max_inflight) break;
-            const int64_t q_slot = (r->kind == RK_LOOKUP && r->pos_off < 0) ? r->count : 0;
+            const int64_t q_slot = (r->kind == RK_LOOKUP && !r->dpos && r->pos_off < 0) ? r->count : 0;
             if (r->kind == RK_LOOKUP &&
                 (!slot || R + 1 > glc::kServeMaxReqs || slot_need(R + 1, Q + q_slot) > s->slot_bytes)) {
                 ++it;  // next step
@@ -372,6 +383,7 @@ int admit_and_launch(glc_server* s) {
         s->inflight -= static_cast<int32_t>(rs.size());
         s->cv_done.notify_all();
     };
+    *launched = static_cast<int>(look.size() + dec.size());
     int first_rc = GLC_OK;
     if (!look.empty()) {
         const int rc = launch_step(s, *slot, look);
@@ -391,15 +403,16 @@ int admit_and_launch(glc_server* s) {
 }
 
 void worker_main(glc_server* s) {
-    // The worker polls in-flight steps with short sleeps; the default 50 us
-    // timer slack would stretch each 5 us sleep to ~55 us and delay every
-    // completion by that much.  1 us slack, for this thread only.
+    // With polling switched off (GLC_SERVE_SPIN_US=0) the worker sleeps 5 us
+    // between polls of in-flight work; the default 50 us timer slack would
+    // stretch each sleep to ~55 us.  1 us slack, for this thread only.
     (void)prctl(PR_SET_TIMERSLACK, 1000UL, 0UL, 0UL, 0UL);
     if (hipSetDevice(s->device) != hipSuccess) {
         std::lock_guard<std::mutex> lk(s->mu);
         s->worker_rc = GLC_ERR_HIP;
         s->worker_err = "hipSetDevice failed in the serving worker";
     }
+    double last_active = now_s();
     for (;;) {
         bool stopping;
         {
@@ -421,25 +434,38 @@ void worker_main(glc_server* s) {
             }
             if (!stopping && s->inflight == 0 && s->queue.empty() && s->to_free.empty()) {
                 // idle, or waiting for a client to release results
-                s->cv_work.wait_for(lk, std::chrono::milliseconds(s->pending.empty() ? 50 : 1));
-                if (s->queue.empty() && s->to_free.empty()) continue;
+                if (now_s() - last_active >= s->spin_s) {
+                    s->cv_work.wait_for(lk, std::chrono::milliseconds(s->pending.empty() ? 50 : 1));
+                    if (s->queue.empty() && s->to_free.empty()) continue;
+                } else {
+                    // poll the submission / release counter without the lock
+                    const int64_t seq = s->work_seq.load(std::memory_order_acquire);
+                    lk.unlock();
+                    while (s->work_seq.load(std::memory_order_acquire) == seq && now_s() - last_active < s->spin_s)
+                        __builtin_ia32_pause();
+                    continue;
+                }
             }
         }
-        retire(s);
+        const int done = retire(s);
+        int launched = 0;
         if (!stopping) {
-            const int rc = admit_and_launch(s);
+            const int rc = admit_and_launch(s, &launched);
             if (rc) {
                 std::lock_guard<std::mutex> lk(s->mu);
                 s->worker_rc = rc;
                 s->worker_err = glc_last_error();
             }
         }
-        {
+        if (done || launched) last_active = now_s();
+        if (s->spin_s <= 0.0) {
             std::unique_lock<std::mutex> lk(s->mu);
             if (s->inflight > 0 && s->queue.empty()) {
                 lk.unlock();
                 std::this_thread::sleep_for(std::chrono::microseconds(5));
             }
+        } else if (!done && !launched) {
+            __builtin_ia32_pause();
         }
     }
 }
@@ -459,7 +485,7 @@ int submit(glc_server* s, Req* r, int64_t* out_id) {
         return fail(code, msg);
     };
     if (glc::align_up(r->out_bytes, kAlign) > s->dev.size() ||
-        (r->kind == RK_LOOKUP && slot_need(1, r->count) > s->slot_bytes))
+        (r->kind == RK_LOOKUP && slot_need(1, r->dpos ? 0 : r->count) > s->slot_bytes))
         return drop(GLC_ERR_BUDGET, "request needs more memory than the serving budget / staging allows");
     if (s->stop) return drop(GLC_ERR_STATE, "server is shutting down");
     r->id = s->next_id++;
@@ -467,6 +493,7 @@ int submit(glc_server* s, Req* r, int64_t* out_id) {
     s->reqs.emplace(r->id, std::unique_ptr<Req>(r));
     s->queue.push_back(r);
     *out_id = r->id;
+    s->work_seq.fetch_add(1, std::memory_order_release);
     s->cv_work.notify_one();
     return GLC_OK;
 }
@@ -570,6 +597,7 @@ int glc_server_create(int64_t hbm_budget_bytes, int64_t staging_bytes, int32_t m
         shutdown(s);
         return fail(GLC_ERR_HIP, "stream creation failed");
     }
+    if (const char* e = std::getenv("GLC_SERVE_SPIN_US")) s->spin_s = std::max(0.0, std::atof(e)) * 1e-6;
     s->worker = std::thread(worker_main, s);
     *out = s;
     return GLC_OK;
@@ -613,6 +641,27 @@ int glc_server_submit_lookup(glc_server* s, int32_t col_id, const int32_t* h_pos
     } else {
         r->pos.assign(h_pos, h_pos + q);
     }
+    return submit(s, r, req_id);
+}
+
+int glc_server_submit_lookup_device(glc_server* s, int32_t col_id, const int32_t* d_pos, int64_t q,
+                                    int64_t* req_id) {
+    if (!s || !req_id || !d_pos) return fail(GLC_ERR_INVALID_ARG, "null argument");
+    if (q < 1 || q > (1LL << 28)) return fail(GLC_ERR_INVALID_ARG, "lookup count must be in [1, 2^28]");
+    const glc_column* c;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (col_id < 0 || col_id >= static_cast<int32_t>(s->cols.size()))
+            return fail(GLC_ERR_INVALID_ARG, "unknown column id");
+        c = s->cols[col_id].c;
+    }
+    auto* r = new Req();
+    r->kind = RK_LOOKUP;
+    r->col = col_id;
+    r->c = c;
+    r->count = q;
+    r->out_bytes = q * c->elem_size;
+    r->dpos = d_pos;
     return submit(s, r, req_id);
 }
 
@@ -686,6 +735,7 @@ int glc_server_release(glc_server* s, int64_t req_id) {
     if (r->state != RQ_DONE && r->state != RQ_FAILED) return fail(GLC_ERR_STATE, "request not finished");
     if (r->out_off >= 0) s->to_free.emplace_back(r->out_off, r->out_bytes);
     s->reqs.erase(req_id);
+    s->work_seq.fetch_add(1, std::memory_order_release);
     s->cv_work.notify_one();
     return GLC_OK;
 }

@@ -96,6 +96,7 @@ SIGNATURES = [
     ("glc_server_create", _int, [_i64, _i64, _i32, _vp]),
     ("glc_server_add_column", _int, [_vp, _vp, _vp]),
     ("glc_server_submit_lookup", _int, [_vp, _i32, _vp, _i64, _vp]),
+    ("glc_server_submit_lookup_device", _int, [_vp, _i32, _vp, _i64, _vp]),
     ("glc_server_submit_decompress", _int, [_vp, _i32, _vp]),
     ("glc_server_wait", _int, [_vp, _i64, _i32]),
     ("glc_server_result", _int, [_vp, _i64, _vp, _vp, _vp]),
@@ -584,6 +585,15 @@ class Server:
         rid = C.c_int64()
         _check(lib().glc_server_submit_lookup(self._h, col_id, pos.ctypes.data_as(C.c_void_p), pos.size,
                                                C.byref(rid)))
+        return rid.value
+
+    def submit_lookup_device(self, col_id: int, d_pos) -> int:
+        """Lookup of int32 positions already on the device (read in place; keep
+        d_pos alive and unchanged until the request is done)."""
+        if d_pos.dtype.itemsize != 4:
+            raise ValueError("positions must be int32")
+        rid = C.c_int64()
+        _check(lib().glc_server_submit_lookup_device(self._h, col_id, _devptr(d_pos), d_pos.numel(), C.byref(rid)))
         return rid.value
 
     def submit_decompress(self, col_id: int) -> int:

@@ -303,6 +303,12 @@ int glc_server_add_column(glc_server* s, const glc_column* col, int32_t* col_id)
 /* Queue q lookups of column col_id (positions copied; out-of-range -> 0).
  * GLC_ERR_BUDGET when the request can never fit the budget/staging. */
 int glc_server_submit_lookup(glc_server* s, int32_t col_id, const int32_t* h_pos, int64_t q, int64_t* req_id);
+/* Queue q lookups whose positions are already in device memory (written by
+ * an earlier kernel, for instance): the lookup kernel reads them in place,
+ * so nothing is copied; d_pos must stay valid and unchanged until the
+ * request is done, and the work that wrote it must be complete when the
+ * call is made. */
+int glc_server_submit_lookup_device(glc_server* s, int32_t col_id, const int32_t* d_pos, int64_t q, int64_t* req_id);
 /* Queue a full decompression of column col_id. */
 int glc_server_submit_decompress(glc_server* s, int32_t col_id, int64_t* req_id);
 /* Wait for a request (timeout_ms < 0: no limit; GLC_ERR_STATE on timeout). */

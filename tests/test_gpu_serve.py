@@ -152,3 +152,41 @@ def test_lookups_when_the_position_arena_is_full(gpu):
     st = srv.stats()
     assert st["completed"] == len(reqs) and st["worker_status"] == 0
     srv.close()
+
+
+def test_device_resident_positions(gpu):
+    # glc_server_submit_lookup_device: positions read in place from device
+    # memory give the same results as host positions and as the plain path,
+    # mixed with host submissions and decompressions
+    vals, cols = columns()
+    srv = glc.Server(256 << 20, max_inflight=512)
+    ids = [srv.add_column(c) for c in cols]
+    rng = np.random.default_rng(21)
+    keep, reqs = [], []
+    for k in range(120):
+        c = int(rng.integers(0, len(vals)))
+        n = vals[c].size
+        q = int(rng.choice([1, 63, 1000, 4097, 20000]))
+        pos = rng.integers(-3, n + 3, q).astype(np.int32)
+        d = torch.from_numpy(pos).cuda()
+        keep.append(d)
+        torch.cuda.synchronize()
+        kind = k % 3
+        if kind == 0:
+            rid = srv.submit_lookup_device(ids[c], d)
+        elif kind == 1:
+            rid = srv.submit_lookup(ids[c], pos)
+        else:
+            rid = srv.submit_decompress(ids[c])
+        reqs.append((rid, c, pos, kind))
+    for rid, c, pos, kind in reqs:
+        srv.wait(rid, 60000)
+        got = srv.result(rid, ids[c])
+        want = expected(vals, "dec" if kind == 2 else "look", c, pos)
+        assert np.array_equal(got, want), (kind, c)
+        srv.release(rid)
+    st = srv.stats()
+    assert st["completed"] == len(reqs) and st["worker_status"] == 0
+    with pytest.raises(glc.GlcError):
+        srv.submit_lookup_device(ids[0], torch.zeros(0, dtype=torch.int32, device="cuda"))
+    srv.close()
