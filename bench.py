@@ -34,6 +34,8 @@ META_BYTES_PER_PARTITION = 36
 # result (coalesced), one 64-B sector of the 32-B tile record and one 64-B
 # sector of the payload word(s) it decodes.
 LOOKUP_LINE_BYTES = 4 + 8 + 64 + 64
+SCHEDULE_NAMES = {0: "tile per one-wave workgroup", 1: "4 tiles per four-wave workgroup",
+                  2: "half tile per one-wave workgroup"}
 # bench --mode -> the decode kernel that mode launches
 DECODER_KERNEL = {
     "worksteal": "k_decode_wave (GLC_FULL_WORKSTEAL: one wave per 1024-value tile, tiles handed to CUs by the "
@@ -273,6 +275,7 @@ def bench_i32(args):
     torch.cuda.synchronize()
     inf = col.info()
     out = torch.empty_like(x)
+    sched, _ = col.autotune_decode(out, reps=10)
     for _ in range(args.warmup):
         col.decompress_full(glc.FULL_BITPACKED, out=out)
     torch.cuda.synchronize()
@@ -291,7 +294,8 @@ def bench_i32(args):
         "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "int32", "data": "synthetic (seeded, generated on the GPU)",
         "config": {"workload": f"int32_linear_noise13_N2^{args.log2n}_fixed4096", "n": n,
-                   "partitions": inf.num_partitions, "compressed_bytes": inf.compressed_size},
+                   "partitions": inf.num_partitions, "compressed_bytes": inf.compressed_size,
+                   "decode_schedule": SCHEDULE_NAMES[sched]},
         "roofline": {"bound": "hbm", "achieved": round(alg / (ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "k_decode_wave (int32, bit-packed)", "algorithmic_bytes": alg},
@@ -333,6 +337,7 @@ def bench_c3(args):
         torch.cuda.synchronize()
         return sum(a.elapsed_time(c) for a, c in ev) / steps
 
+    sched, _ = col.autotune_decode(out, reps=10)
     res = {}
     res["packed_full_ms"] = timed(lambda: col.decompress_full(glc.FULL_BITPACKED, out=out), args.steps)
     assert torch.equal(out, x)
@@ -359,7 +364,8 @@ def bench_c3(args):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int64",
         "data": "synthetic (seeded, generated on the GPU)",
         "config": {"workload": f"c3_int64_sorted_timestamps_N2^{args.log2n}_fixed4096", "n": n, "partitions": P,
-                   "compressed_bytes": inf.compressed_size, "lookups": q, "lookup_mix": "uniform"},
+                   "compressed_bytes": inf.compressed_size, "lookups": q, "lookup_mix": "uniform",
+                   "decode_schedule": SCHEDULE_NAMES[sched]},
         "ms": {k: round(v, 4) for k, v in res.items()},
         "full_GBps": {"packed": round(8 * n / res["packed_full_ms"] / 1e6, 1),
                       "packed_fix": round(8 * n / res["packed_fix_full_ms"] / 1e6, 1),
@@ -818,6 +824,10 @@ def main():
     mode = {"worksteal": glc.FULL_WORKSTEAL, "bitpacked": glc.FULL_BITPACKED, "v2": glc.FULL_V2,
             "worksteal_adv": glc.FULL_WORKSTEAL_ADV}[args.mode]
 
+    # the column's decode schedule, measured on this column (setup, like the
+    # batched form's autotuner)
+    sched, sched_ms = col.autotune_decode(out, reps=10)
+    log(f"decode schedule {SCHEDULE_NAMES[sched]} ({sched_ms:.4f} ms median in the tuner)")
     for _ in range(args.warmup):
         col.decompress_full(mode, out=out)
         col.random_access(pos, out=look)
@@ -921,7 +931,8 @@ def main():
         "dtype": "u64",
         "data": "synthetic (seeded generator of the configured shape, generated on the GPU)",
         "config": {"workload": f"c2_uint64_piecewise_linear_N2^{args.log2n}_v6", "n": n, "partitioner": "v6",
-                   "v6_base": 2048, "decoder": DECODER_KERNEL[args.mode], "lookups_per_step": q,
+                   "v6_base": 2048, "decoder": DECODER_KERNEL[args.mode], "decode_schedule": SCHEDULE_NAMES[sched],
+                   "lookups_per_step": q,
                    "lookup_mix": "1/2 uniform + 1/2 zipf(1.1) scrambled", "partitions": inf.num_partitions,
                    "compressed_bytes": inf.compressed_size, "compression_ratio": round(n * 8 / inf.compressed_size, 4)},
         "decode_ms_avg": round(dec_avg, 4),

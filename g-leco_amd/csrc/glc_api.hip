@@ -686,6 +686,27 @@ int glc_decompress_full(const glc_column* c, void* d_out, int mode, void* stream
     return launch_decode_full(c, d_out, mode, static_cast<hipStream_t>(stream));
 }
 
+int glc_column_set_decode_schedule(glc_column* c, int32_t schedule) {
+    if (!c) return fail(GLC_ERR_INVALID_ARG, "NULL column");
+    if (schedule < GLC_DECODE_TILE_W1 || schedule > GLC_DECODE_HALF_TILE_W1)
+        return fail(GLC_ERR_INVALID_ARG, "unknown decode schedule");
+    c->dec_sched = schedule;
+    return GLC_OK;
+}
+
+int glc_column_get_decode_schedule(const glc_column* c, int32_t* schedule) {
+    if (!c || !schedule) return fail(GLC_ERR_INVALID_ARG, "NULL argument");
+    *schedule = c->dec_sched;
+    return GLC_OK;
+}
+
+int glc_column_autotune_decode(glc_column* c, void* d_out, int32_t reps, void* stream, int32_t* schedule, float* ms) {
+    if (!c) return fail(GLC_ERR_INVALID_ARG, "NULL column");
+    if (c->n > 0 && !d_out) return fail(GLC_ERR_INVALID_ARG, "NULL output");
+    if (reps < 1 || reps > 1000) return fail(GLC_ERR_INVALID_ARG, "reps must be in [1, 1000]");
+    return glc_autotune_decode(c, d_out, reps, static_cast<hipStream_t>(stream), schedule, ms);
+}
+
 int glc_decompress_full_fixed(const glc_column* c, void* d_out, int32_t partition_size, void* stream) {
     if (!c) return fail(GLC_ERR_INVALID_ARG, "NULL column");
     if (c->n > 0 && !d_out) return fail(GLC_ERR_INVALID_ARG, "NULL output");

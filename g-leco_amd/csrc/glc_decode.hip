@@ -18,6 +18,9 @@
 //                       stores its elements.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <vector>
+
 #include "glc_device.hpp"
 #include "glc_internal.hpp"
 
@@ -239,28 +242,34 @@ __device__ __forceinline__ void wave_narrow(const DecArgs& a, const TileInfo& x,
 // words of the row window (words wb .. wb+128, as wave_narrow's rows).
 // Fewer, wider stores per wave: 8 instead of 16 for 64-bit values, 4
 // instead of 16 for 32-bit values with fields of at most 16 bits.
-template <class T, int OP, int V>
-__device__ __forceinline__ void wave_narrow_v(const DecArgs& a, const TileInfo& x, int64_t e0, int64_t e1, int lane) {
+// PART = 0: the whole tile; PART = 1: rows h*half .. of the tile's two
+// halves (the half-tile schedule, one wave per half).
+template <class T, int OP, int V, int PART = 0>
+__device__ __forceinline__ void wave_narrow_v(const DecArgs& a, const TileInfo& x, int64_t e0, int64_t e1, int lane,
+                                              int h = 0) {
     typedef typename std::conditional<OP == OP_UNPACK, int64_t, T>::type O;  // output element
     constexpr int kRowVals = 64 * V;
-    constexpr int kRows = kWaveTile / kRowVals;
+    constexpr int kRows = kWaveTile / kRowVals / (PART ? 2 : 1);
     static_assert(V * sizeof(O) == 16 || V * sizeof(O) == 8, "16- or 8-byte stores");
+    const int rbase = PART ? h * kRows : 0;
     const int b = tile_bits(x.info);
     const uint32_t* __restrict__ w = a.words;
     uint32_t x0[kRows], x1[kRows], xs[kRows];
 #pragma unroll
-    for (int r = 0; r < kRows; ++r) {  // branch-free, clamped as in wave_narrow
+    for (int rr = 0; rr < kRows; ++rr) {  // branch-free, clamped as in wave_narrow
+        const int r = rbase + rr;
         const int64_t R = x.W + static_cast<int64_t>(r * kRowVals) * b;
         const int64_t wb = (R >> 5) < a.wlimit ? (R >> 5) : a.wlimit;
-        x0[r] = w[wb + lane];
-        x1[r] = w[wb + 64 + lane];
-        xs[r] = w[wb + 128];
+        x0[rr] = w[wb + lane];
+        x1[rr] = w[wb + 64 + lane];
+        xs[rr] = w[wb + 128];
     }
     const int32_t lbase = x.lbase;
     const uint32_t rel_lane = static_cast<uint32_t>(lane) * static_cast<uint32_t>(V * b);
     O* __restrict__ out = static_cast<O*>(a.out);
 #pragma unroll
-    for (int r = 0; r < kRows; ++r) {
+    for (int rr = 0; rr < kRows; ++rr) {
+        const int r = rbase + rr;
         const int64_t R = x.W + static_cast<int64_t>(r * kRowVals) * b;
         const uint32_t rel = static_cast<uint32_t>(R & 31) + rel_lane;
         const uint32_t wi = rel >> 5, sh = rel & 31;
@@ -268,8 +277,8 @@ __device__ __forceinline__ void wave_narrow_v(const DecArgs& a, const TileInfo& 
 #pragma unroll
         for (int k = 0; k < 3; ++k) {  // words wi .. wi+2 of the row window (wi + 2 <= 128)
             const uint32_t j = wi + k;
-            const uint32_t p0 = bperm(x0[r], j), p1 = bperm(x1[r], j);
-            g[k] = (j < 64) ? p0 : (j < 128) ? p1 : xs[r];
+            const uint32_t p0 = bperm(x0[rr], j), p1 = bperm(x1[rr], j);
+            g[k] = (j < 64) ? p0 : (j < 128) ? p1 : xs[rr];
         }
         O v[V];
 #pragma unroll
@@ -382,6 +391,59 @@ __device__ __forceinline__ void wave_plain(const DecArgs& a, const TileInfo& x, 
     }
 }
 
+// Pre-unpacked deltas, two per lane: one 16-byte load of two adjacent int64
+// deltas and one 16-byte (8-byte for 32-bit values) store per lane and row
+// of 128 values.  The delta array is a device allocation (16-byte aligned)
+// and e is even, so the vector loads are aligned; the output's alignment is
+// the caller's (VEC instantiation).
+template <class T>
+__device__ __forceinline__ void wave_plain_v(const DecArgs& a, const TileInfo& x, int64_t e0, int64_t e1, int lane) {
+    constexpr int kRows = kWaveTile / 128;
+    typedef int64_t i64x2 __attribute__((ext_vector_type(2)));
+    i64x2 pl[kRows];
+#pragma unroll
+    for (int r = 0; r < kRows; ++r) {
+        const int64_t e = e0 + r * 128 + 2 * lane;
+        if (e + 2 <= e1) {
+            pl[r] = *reinterpret_cast<const i64x2*>(a.plain + e);
+        } else {
+            pl[r].x = (e < e1) ? a.plain[e] : 0;
+            pl[r].y = 0;
+        }
+    }
+    const int32_t type = tile_type(x.info);
+    const int32_t lbase = x.lbase;
+    T* __restrict__ out = static_cast<T*>(a.out);
+#pragma unroll
+    for (int r = 0; r < kRows; ++r) {
+        const int64_t e = e0 + r * 128 + 2 * lane;
+        const int32_t i = lbase + r * 128 + 2 * lane;
+        T v[2];
+        v[0] = (type == MODEL_DIRECT_COPY) ? static_cast<T>(pl[r].x) : apply_delta<T>(dec_pred<T>(x.t0, x.t1, i), pl[r].x);
+        v[1] = (type == MODEL_DIRECT_COPY) ? static_cast<T>(pl[r].y)
+                                           : apply_delta<T>(dec_pred<T>(x.t0, x.t1, i + 1), pl[r].y);
+        if (e + 2 <= e1) {
+            if constexpr (sizeof(T) == 8) {
+                typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+                u32x4 q;
+                q.x = static_cast<uint32_t>(static_cast<uint64_t>(v[0]));
+                q.y = static_cast<uint32_t>(static_cast<uint64_t>(v[0]) >> 32);
+                q.z = static_cast<uint32_t>(static_cast<uint64_t>(v[1]));
+                q.w = static_cast<uint32_t>(static_cast<uint64_t>(v[1]) >> 32);
+                put(reinterpret_cast<u32x4*>(out + e), q);
+            } else {
+                typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+                u32x2 q;
+                q.x = static_cast<uint32_t>(v[0]);
+                q.y = static_cast<uint32_t>(v[1]);
+                put(reinterpret_cast<u32x2*>(out + e), q);
+            }
+        } else if (e < e1) {
+            put(out + e, v[0]);
+        }
+    }
+}
+
 // Slow path: a tile spanning several partitions, gaps, or a decode-only
 // polynomial model.  Per-lane partition search inside the tile's candidate
 // range, direct extraction from global memory.
@@ -435,7 +497,10 @@ __device__ __forceinline__ void wave_tile(const DecArgs& a, int64_t t, int lane)
     const TileInfo x = a.tinfo[t];
     const int path = tile_path(x.info);
     if constexpr (OP == OP_PLAIN) {
-        if (path != TILE_SLOW) wave_plain<T>(a, x, e0, e1, lane);
+        if (path != TILE_SLOW) {
+            if constexpr (VEC) wave_plain_v<T>(a, x, e0, e1, lane);
+            else wave_plain<T>(a, x, e0, e1, lane);
+        }
         else wave_slow<T, OP>(a, e0, e1, a.tile_part[t], lane);
     } else {
         if (path == TILE_NARROW) {
@@ -456,13 +521,34 @@ __device__ __forceinline__ void wave_tile(const DecArgs& a, int64_t t, int lane)
 // the uint64 2^28 column: 0.410 ms against 0.422-0.431 ms with four waves
 // (four consecutive tiles) per workgroup; the plain write of 2 GiB in 8-KB
 // per-wave runs takes 0.370 ms against 0.397 ms the same way.
-constexpr int kDecodeWaveWpg = 1;
-template <class T, int OP, bool VEC>
-__global__ __launch_bounds__(64 * kDecodeWaveWpg) void k_decode_wave(DecArgs a) {
+template <class T, int OP, bool VEC, int WPG>
+__global__ __launch_bounds__(64 * WPG) void k_decode_wave(DecArgs a) {
     const int lane = threadIdx.x & 63;
-    const int64_t t = blockIdx.x;
+    const int wave = (WPG == 1) ? 0 : __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * WPG + wave;
     if (t * kWaveTile >= a.n) return;
     wave_tile<T, OP, VEC>(a, t, lane);
+}
+
+// Half tiles (packed decode): one-wave workgroup g decodes half g & 1 of
+// narrow tile g >> 1; a tile of any other path is decoded whole by its
+// first half's wave.
+template <class T, bool VEC>
+__global__ __launch_bounds__(64) void k_decode_half(DecArgs a, int64_t ntiles) {
+    const int lane = threadIdx.x & 63;
+    const int64_t t = static_cast<int64_t>(blockIdx.x) >> 1;
+    const int h = static_cast<int>(blockIdx.x & 1);
+    if (t >= ntiles) return;
+    const TileInfo x = a.tinfo[t];
+    if (tile_path(x.info) != TILE_NARROW || !VEC) {
+        if (h == 0) wave_tile<T, OP_PACKED, VEC>(a, t, lane);
+        return;
+    }
+    const int64_t e0 = t * kWaveTile;
+    const int64_t e1 = (e0 + kWaveTile < a.n) ? e0 + kWaveTile : a.n;
+    if constexpr (sizeof(T) == 8) wave_narrow_v<T, OP_PACKED, 2, 1>(a, x, e0, e1, lane, h);
+    else if (tile_bits(x.info) <= 16) wave_narrow_v<T, OP_PACKED, 4, 1>(a, x, e0, e1, lane, h);
+    else wave_narrow_v<T, OP_PACKED, 2, 1>(a, x, e0, e1, lane, h);
 }
 
 // Work stealing.  The tiles form 8 contiguous queues, one per XCD (the
@@ -592,16 +678,22 @@ DecArgs make_args(const glc_column* c, void* out) {
     return a;
 }
 
-template <class T, int OP> int run(const DecArgs& a, int sched, hipStream_t s) {
+template <class T, int OP, bool VEC> void launch_static(const DecArgs& a, int dsched, int64_t nwt, hipStream_t s) {
+    if (dsched == GLC_DECODE_HALF_TILE_W1 && OP == OP_PACKED)
+        hipLaunchKernelGGL((k_decode_half<T, VEC>), dim3(static_cast<unsigned>(2 * nwt)), dim3(64), 0, s, a, nwt);
+    else if (dsched == GLC_DECODE_TILE_W1 || dsched == GLC_DECODE_HALF_TILE_W1)
+        hipLaunchKernelGGL((k_decode_wave<T, OP, VEC, 1>), dim3(static_cast<unsigned>(nwt)), dim3(64), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_decode_wave<T, OP, VEC, 4>), dim3(static_cast<unsigned>((nwt + 3) / 4)), dim3(256), 0, s,
+                           a);
+}
+
+template <class T, int OP> int run(const DecArgs& a, int sched, hipStream_t s, int dsched = GLC_DECODE_TILE_W4) {
     const int64_t nwt = (a.n + kWaveTile - 1) / kWaveTile;
     switch (sched) {
         case 0:
-            if (a.vec16)
-                hipLaunchKernelGGL((k_decode_wave<T, OP, true>), dim3(static_cast<unsigned>(nwt)), dim3(64 * kDecodeWaveWpg),
-                                   0, s, a);
-            else
-                hipLaunchKernelGGL((k_decode_wave<T, OP, false>), dim3(static_cast<unsigned>(nwt)),
-                                   dim3(64 * kDecodeWaveWpg), 0, s, a);
+            if (a.vec16) launch_static<T, OP, true>(a, dsched, nwt, s);
+            else launch_static<T, OP, false>(a, dsched, nwt, s);
             break;
         case 1: {
             hipLaunchKernelGGL(k_zero_queues, dim3(1), dim3(kStealQueues * kQueueStride), 0, s, a.sched);
@@ -668,8 +760,65 @@ int launch_decode_full(const glc_column* c, void* d_out, int mode, hipStream_t s
     DecArgs a = make_args(c, d_out);
     return dispatch_dtype(c->dtype, [&](auto t) {
         using T = decltype(t);
-        return plain ? run<T, OP_PLAIN>(a, sched, s) : run<T, OP_PACKED>(a, sched, s);
+        return plain ? run<T, OP_PLAIN>(a, sched, s, c->dec_sched) : run<T, OP_PACKED>(a, sched, s, c->dec_sched);
     });
+}
+
+int glc_autotune_decode(glc_column* c, void* d_out, int32_t reps, hipStream_t s, int32_t* best_out, float* ms_out) {
+    if (c->n == 0 || c->P == 0 || (!c->d_words && c->total_bits > 0)) {
+        if (best_out) *best_out = c->dec_sched;
+        if (ms_out) *ms_out = 0.0f;
+        return GLC_OK;
+    }
+    constexpr int kS = 3;
+    const int sched_of[kS] = {GLC_DECODE_TILE_W1, GLC_DECODE_TILE_W4, GLC_DECODE_HALF_TILE_W1};
+    const int saved = c->dec_sched;
+    std::vector<hipEvent_t> ev(static_cast<size_t>(2 * kS * reps), nullptr);
+    int rc = GLC_OK;
+    for (auto& e : ev)
+        if (hipEventCreate(&e) != hipSuccess) {
+            e = nullptr;
+            rc = fail(GLC_ERR_HIP, "hipEventCreate failed");
+            break;
+        }
+    // one warm-up launch per schedule, then `reps` rounds over the schedules
+    // (round-robin, so clock and cache drift falls on every schedule alike)
+    for (int k = 0; k < kS && !rc; ++k) {
+        c->dec_sched = sched_of[k];
+        rc = launch_decode_full(c, d_out, GLC_FULL_BITPACKED, s);
+    }
+    for (int r = 0; r < reps && !rc; ++r)
+        for (int k = 0; k < kS && !rc; ++k) {
+            c->dec_sched = sched_of[k];
+            const size_t i = static_cast<size_t>(2 * (r * kS + k));
+            if (hipEventRecord(ev[i], s) != hipSuccess) rc = fail(GLC_ERR_HIP, "hipEventRecord failed");
+            if (!rc) rc = launch_decode_full(c, d_out, GLC_FULL_BITPACKED, s);
+            if (!rc && hipEventRecord(ev[i + 1], s) != hipSuccess) rc = fail(GLC_ERR_HIP, "hipEventRecord failed");
+        }
+    if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = fail(GLC_ERR_HIP, "hipStreamSynchronize failed");
+    int best = saved;
+    float best_ms = 0.0f;
+    if (!rc) {
+        for (int k = 0; k < kS; ++k) {  // median launch time of each schedule
+            std::vector<float> t(static_cast<size_t>(reps));
+            for (int r = 0; r < reps; ++r) {
+                const size_t i = static_cast<size_t>(2 * (r * kS + k));
+                (void)hipEventElapsedTime(&t[static_cast<size_t>(r)], ev[i], ev[i + 1]);
+            }
+            std::sort(t.begin(), t.end());
+            const float med = t[t.size() / 2];
+            if (k == 0 || med < best_ms) {
+                best_ms = med;
+                best = sched_of[k];
+            }
+        }
+    }
+    for (auto& e : ev)
+        if (e) (void)hipEventDestroy(e);
+    c->dec_sched = rc ? saved : best;
+    if (!rc && best_out) *best_out = best;
+    if (!rc && ms_out) *ms_out = best_ms;
+    return rc;
 }
 
 int launch_batch_decode(const BatchPlan& b, int wpg, int tpw, hipStream_t s) {

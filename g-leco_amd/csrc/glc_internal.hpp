@@ -176,6 +176,7 @@ struct glc_column {
     int32_t fit_block = 0;
     int32_t fit_form = 0;           // contraction form of the fit (glc_opts.fit_form)
     int32_t has_poly = 0;
+    int32_t dec_sched = GLC_DECODE_TILE_W4;  // packed full-decode schedule (glc_column_set_decode_schedule)
     // arena
     void* meta_arena = nullptr;
     int64_t meta_bytes = 0;
@@ -280,6 +281,7 @@ int v6_capacity(int64_t n, int32_t base, int64_t* cap, int64_t* nvb);
 int launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s);
 // decode side (glc_decode.hip)
 int launch_decode_full(const glc_column* c, void* d_out, int mode, hipStream_t s);
+int glc_autotune_decode(glc_column* c, void* d_out, int32_t reps, hipStream_t s, int32_t* best, float* ms);
 int launch_preunpack(glc_column* c, int64_t* d_plain, hipStream_t s);
 // access side (glc_access.hip)
 int launch_random_access(const glc_column* c, const void* d_pos, int pos64, int64_t q, void* d_out, int mode,

@@ -24,6 +24,7 @@ LIB_PATH = os.path.normpath(os.path.join(_HERE, "..", "lib", "libglc.so"))
 I32, U32, I64, U64 = 0, 1, 2, 3
 PART_FIXED, PART_V6, PART_CPU_SPLITMERGE, PART_GPU_SPLITMERGE, PART_EXPLICIT = 0, 1, 2, 3, 4
 FULL_AUTO, FULL_BITPACKED, FULL_PREUNPACKED, FULL_V2, FULL_WORKSTEAL, FULL_WORKSTEAL_ADV, FULL_COOP_EXACT = range(7)
+DECODE_TILE_W1, DECODE_TILE_W4, DECODE_HALF_TILE_W1 = range(3)
 RA_GENERIC, RA_PREUNPACKED = 0, 1
 ERR_NAMES = {0: "OK", -1: "INVALID_ARG", -2: "HIP", -3: "OOM", -4: "FORMAT", -5: "UNSUPPORTED", -6: "BUDGET",
              -7: "STATE"}
@@ -67,6 +68,9 @@ SIGNATURES = [
     ("glc_partition_v6_noalloc", _int, [_int, _vp, _i64, _i32, _vp, _vp, _i64, _vp, _vp]),
     ("glc_decompress_full", _int, [_vp, _vp, _int, _vp]),
     ("glc_decompress_full_fixed", _int, [_vp, _vp, _i32, _vp]),
+    ("glc_column_set_decode_schedule", _int, [_vp, _i32]),
+    ("glc_column_get_decode_schedule", _int, [_vp, _vp]),
+    ("glc_column_autotune_decode", _int, [_vp, _vp, _i32, _vp, _vp, _vp]),
     ("glc_random_access", _int, [_vp, _vp, _i64, _vp, _int, _vp]),
     ("glc_random_access_i64", _int, [_vp, _vp, _i64, _vp, _int, _vp]),
     ("glc_random_access_fixed", _int, [_vp, _vp, _i64, _vp, _i32, _int, _vp]),
@@ -290,6 +294,26 @@ class Column:
         dpos = torch.empty(q, dtype=torch.int32, device="cuda")
         copy_to_device(dpos, pos, stream)
         return copy_to_host(out, self.random_access(dpos, mode, stream=stream), stream)
+
+    def set_decode_schedule(self, schedule: int) -> None:
+        _check(lib().glc_column_set_decode_schedule(self._h, schedule))
+
+    def decode_schedule(self) -> int:
+        v = C.c_int32()
+        _check(lib().glc_column_get_decode_schedule(self._h, C.byref(v)))
+        return v.value
+
+    def autotune_decode(self, out=None, reps: int = 10, stream=None):
+        """Time the packed full-decode schedules on this column (into `out`)
+        and keep the fastest; returns (schedule, median ms)."""
+        import torch
+        n = self.n
+        if out is None:
+            out = torch.empty(n, dtype=_torch_dtype(self.dtype), device="cuda")
+        sch, ms = C.c_int32(), C.c_float()
+        _check(lib().glc_column_autotune_decode(self._h, _devptr(out) if n else None, reps, _stream(stream),
+                                                C.byref(sch), C.byref(ms)))
+        return sch.value, ms.value
 
     def decompress_full_fixed(self, partition_size: int, out=None, stream=None):
         import torch

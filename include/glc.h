@@ -159,6 +159,24 @@ int glc_partition_v6_noalloc(int dtype, const void* d_values, int64_t n, int32_t
 /* ---- full decompression ---------------------------------------------------
  * d_out: device array of total_values values.  Asynchronous on `stream`.   */
 int glc_decompress_full(const glc_column* col, void* d_out, int mode, void* stream);
+/* Schedule of a column's packed full decode (GLC_FULL_AUTO on a packed
+ * column, BITPACKED, WORKSTEAL and the fixed decode): how the wave tiles are
+ * grouped into workgroups.  It changes only the time, never the output.
+ * Which is fastest depends on the column (its length, widths, element
+ * size); glc_column_autotune_decode measures it.  Default: TILE_W4. */
+enum {
+    GLC_DECODE_TILE_W1 = 0,      /* one 1024-value tile per one-wave workgroup */
+    GLC_DECODE_TILE_W4 = 1,      /* four consecutive tiles per four-wave workgroup */
+    GLC_DECODE_HALF_TILE_W1 = 2  /* one half tile per one-wave workgroup */
+};
+int glc_column_set_decode_schedule(glc_column* col, int32_t schedule);
+int glc_column_get_decode_schedule(const glc_column* col, int32_t* schedule);
+/* Time every schedule on this column -- one warm-up launch each, then `reps`
+ * rounds of one launch per schedule into d_out, each bracketed by HIP events
+ * on `stream` -- and keep the one with the lowest median.  Synchronises the
+ * stream.  schedule / ms (optional) receive the choice and its median time. */
+int glc_column_autotune_decode(glc_column* col, void* d_out, int32_t reps, void* stream, int32_t* schedule,
+                               float* ms);
 /* decompressFullFile_Fix(cd, out, partition_size), G:768-788.  The column
  * must have been partitioned with exactly that fixed size (validated). */
 int glc_decompress_full_fixed(const glc_column* col, void* d_out, int32_t partition_size, void* stream);

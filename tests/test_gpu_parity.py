@@ -459,3 +459,35 @@ def test_parity_fit_form_1(gpu, gen, part):
         assert not np.array_equal(ref0.params.view(np.uint64), ref.params.view(np.uint64))
     with pytest.raises(glc.GlcError):
         glc.compress(to_dev(v[:1000]), gp, size, 0, fit_form=2)
+
+
+@pytest.mark.parametrize("gen", sorted(GENS))
+def test_decode_schedules_and_autotune(gpu, gen):
+    # every packed full-decode schedule gives the same output, on columns
+    # with narrow, wide, DIRECT_COPY and partition-spanning tiles and a
+    # ragged end; the autotuner keeps one of them and the column decodes
+    # exactly with it
+    n = 70_000 + 333
+    v = GENS[gen](n)
+    if v.dtype == np.uint64:
+        v[20000:21000] = np.uint64(1 << 60) + np.arange(1000, dtype=np.uint64)  # DIRECT_COPY
+    for part, size in (("fixed", 4096), ("fixed", 1000), ("v6", 0)):
+        gp, op = PART[part]
+        col = glc.compress(to_dev(v), gp, size if part != "v6" else 0, 2048 if part == "v6" else 0)
+        ref = O.compress(v, op, size, 2048)
+        assert_same_column(col, ref)
+        for sch in (glc.DECODE_TILE_W1, glc.DECODE_TILE_W4, glc.DECODE_HALF_TILE_W1):
+            col.set_decode_schedule(sch)
+            assert col.decode_schedule() == sch
+            for mode in (glc.FULL_BITPACKED, glc.FULL_WORKSTEAL, glc.FULL_AUTO):
+                assert np.array_equal(host(col.decompress_full(mode)), v), (part, sch, mode)
+            # unaligned output: the one-value-per-lane path
+            buf = torch.empty(n + 1, dtype=to_dev(v[:1]).dtype, device="cuda")
+            col.decompress_full(glc.FULL_BITPACKED, out=buf[1:])
+            assert np.array_equal(host(buf[1:]), v), (part, sch, "unaligned")
+        sch, ms = col.autotune_decode(reps=3)
+        assert sch in (glc.DECODE_TILE_W1, glc.DECODE_TILE_W4, glc.DECODE_HALF_TILE_W1) and ms > 0
+        assert col.decode_schedule() == sch
+        assert np.array_equal(host(col.decompress_full()), v)
+    with pytest.raises(glc.GlcError):
+        col.set_decode_schedule(7)
