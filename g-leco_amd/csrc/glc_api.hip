@@ -390,7 +390,8 @@ int compress_impl(int dtype, const void* d_values, int64_t n, int partitioner, i
     GLC_HIP_BAIL(hipGetLastError());
     c->fit_block = fit_block_size(len0);
     c->fit_form = fit_form;
-    if ((rc = launch_fit(dtype, d_values, c, c->fit_block, fit_form, s))) return bail(rc);
+    if ((rc = launch_fit(dtype, d_values, c, c->fit_block, fit_form, partitioner == GLC_PART_FIXED ? S : 0, s)))
+        return bail(rc);
     int64_t* d_total = reinterpret_cast<int64_t*>(summary_ptr(c)) + 8;  // scratch slot after the summary
     if ((rc = launch_scan_offsets(c, d_total, s))) return bail(rc);
     IndexSummary sum;

@@ -56,7 +56,9 @@ struct glc_batch {
 
 namespace {
 
-constexpr int kSchedules[][2] = {{4, 1}, {2, 1}, {8, 1}, {1, 1}, {4, 2}, {2, 4}};
+// {waves per workgroup, 1024-value tiles per wave}; tiles per wave 0 = half
+// a tile per one-wave workgroup
+constexpr int kSchedules[][2] = {{4, 1}, {2, 1}, {8, 1}, {1, 1}, {4, 2}, {2, 4}, {1, 0}};
 
 bool schedule_ok(int w, int t) {
     for (auto& s : kSchedules)

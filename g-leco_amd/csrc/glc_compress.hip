@@ -386,14 +386,14 @@ template <class T, int FORM>
 __global__ __launch_bounds__(64 * kFitWaves) void k_fit_wave(const T* __restrict__ v, const int32_t* __restrict__ st,
                                                             const int32_t* __restrict__ en, int32_t* __restrict__ type,
                                                             double* __restrict__ params, int32_t* __restrict__ bits,
-                                                            int64_t* __restrict__ err, int64_t P) {
+                                                            int64_t* __restrict__ err, int64_t P, bool skip_reg) {
     const int lane = threadIdx.x & 63;
     const int64_t p = static_cast<int64_t>(blockIdx.x) * kFitWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (p >= P) return;
     const int32_t s0 = st[p];
     const int32_t n = en[p] - s0;
     if (n <= 0) return;  // :62, entry keeps its initial values
-    if (n >= kFitRegMin && n <= kFitRegMax) return;  // k_fit_reg's partitions
+    if (skip_reg && n >= kFitRegMin && n <= kFitRegMax) return;  // k_fit_reg's partitions
     fit_wave<T, FORM>(v + s0, n, p, type, params, bits, err, lane);
 }
 
@@ -1279,15 +1279,24 @@ inline unsigned grid_for(int64_t work, int64_t per_block, int64_t cap = 1LL << 3
 
 }  // namespace
 
-template <class T, int FORM> int launch_fit_t(const void* d_values, glc_column* c, int32_t bs, hipStream_t s) {
-    if (bs == kFitWaveBs) {  // two launches, each skipping the other's partitions
-        hipLaunchKernelGGL((k_fit_reg<T, FORM>), dim3(static_cast<unsigned>(c->P)), dim3(256), 0, s,
-                           static_cast<const T*>(d_values), c->d_start, c->d_end, c->d_type, c->d_params, c->d_bits,
-                           c->d_err, c->P);
-        GLC_HIP(hipGetLastError());
+template <class T, int FORM>
+int launch_fit_t(const void* d_values, glc_column* c, int32_t bs, int32_t fixed_size, hipStream_t s) {
+    if (bs == kFitWaveBs) {
+        // fixed partitions of kFitRegMin..kFitRegMax values: the register
+        // form (and k_fit_wave's launch covers only the last, shorter one);
+        // mixed sizes (V6, split/merge): the wave form alone -- two launches
+        // each skipping the other's partitions ran 602 us on the uint64 2^28
+        // V6 column against 555 us for the wave form over all of them
+        if (fixed_size >= kFitRegMin && fixed_size <= kFitRegMax) {
+            hipLaunchKernelGGL((k_fit_reg<T, FORM>), dim3(static_cast<unsigned>(c->P)), dim3(256), 0, s,
+                               static_cast<const T*>(d_values), c->d_start, c->d_end, c->d_type, c->d_params,
+                               c->d_bits, c->d_err, c->P);
+            GLC_HIP(hipGetLastError());
+        }
         const unsigned g = static_cast<unsigned>((c->P + kFitWaves - 1) / kFitWaves);
         hipLaunchKernelGGL((k_fit_wave<T, FORM>), dim3(g), dim3(64 * kFitWaves), 0, s, static_cast<const T*>(d_values),
-                           c->d_start, c->d_end, c->d_type, c->d_params, c->d_bits, c->d_err, c->P);
+                           c->d_start, c->d_end, c->d_type, c->d_params, c->d_bits, c->d_err, c->P,
+                           fixed_size >= kFitRegMin && fixed_size <= kFitRegMax);
         GLC_HIP(hipGetLastError());
         return GLC_OK;
     }
@@ -1298,13 +1307,15 @@ template <class T, int FORM> int launch_fit_t(const void* d_values, glc_column* 
     return GLC_OK;
 }
 
-int launch_fit(int dtype, const void* d_values, glc_column* c, int32_t bs, int fit_form, hipStream_t s) {
+int launch_fit(int dtype, const void* d_values, glc_column* c, int32_t bs, int fit_form, int32_t fixed_size,
+               hipStream_t s) {
     if (c->P == 0) return GLC_OK;
     if (bs < 32 || bs > kFitThreads) return fail(GLC_ERR_INVALID_ARG, "fit block size out of range");
     if (fit_form != 0 && fit_form != 1) return fail(GLC_ERR_INVALID_ARG, "fit_form must be 0 or 1");
     return dispatch_dtype(dtype, [&](auto t) {
         using T = decltype(t);
-        return fit_form == 0 ? launch_fit_t<T, 0>(d_values, c, bs, s) : launch_fit_t<T, 1>(d_values, c, bs, s);
+        return fit_form == 0 ? launch_fit_t<T, 0>(d_values, c, bs, fixed_size, s)
+                             : launch_fit_t<T, 1>(d_values, c, bs, fixed_size, s);
     });
 }
 

@@ -264,7 +264,9 @@ template <class T> __host__ __device__ inline T x86_cast(double r) {
 
 // ---- launchers implemented in the .hip translation units -----------------
 // compress side (glc_compress.hip)
-int launch_fit(int dtype, const void* d_values, glc_column* c, int32_t bs, int fit_form, hipStream_t s);
+// fixed_size: S when the partitions are [iS, min((i+1)S, N)), else 0
+int launch_fit(int dtype, const void* d_values, glc_column* c, int32_t bs, int fit_form, int32_t fixed_size,
+               hipStream_t s);
 int launch_scan_offsets(glc_column* c, int64_t* d_total, hipStream_t s);
 int launch_pack(int dtype, const void* d_values, glc_column* c, hipStream_t s);
 // zero the payload words a compress leaves partly written by atomics (each

@@ -461,7 +461,7 @@ class DirectHandle:
 
 BATCH_COMPRESS = 1
 BATCH_DECOMPRESS = 2
-BATCH_SCHEDULES = [(4, 1), (2, 1), (8, 1), (1, 1), (4, 2), (2, 4)]
+BATCH_SCHEDULES = [(4, 1), (2, 1), (8, 1), (1, 1), (4, 2), (2, 4), (1, 0)]  # (1, 0): half a tile per wave
 
 
 class Batch:

@@ -269,7 +269,8 @@ int glc_batch_decompress(glc_batch* b, void* const* d_out, void* stream);
 enum { GLC_BATCH_COMPRESS = 1, GLC_BATCH_DECOMPRESS = 2 };
 int glc_batch_run_graph(glc_batch* b, int what, void* const* d_out, void* stream);
 /* Decompress schedule: waves per workgroup x 1024-element tiles per wave.
- * Supported: (1,1) (2,1) (4,1) (8,1) (4,2) (2,4). */
+ * Supported: (1,1) (2,1) (4,1) (8,1) (4,2) (2,4), and (1,0): half a tile per
+ * one-wave workgroup. */
 int glc_batch_set_schedule(glc_batch* b, int32_t waves_per_group, int32_t tiles_per_wave);
 int glc_batch_get_schedule(const glc_batch* b, int32_t* waves_per_group, int32_t* tiles_per_wave);
 /* Time every supported schedule on this batch with HIP events (decompress
