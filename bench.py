@@ -549,10 +549,10 @@ def bench_splitmerge(args):
 
     n = 1 << args.log2n
     dev = torch.device("cuda:0")
-    log(f"generating c2 uint64 column on the host, N=2^{args.log2n}")
-    v = W.c2_uint64(n)
-    x = torch.from_numpy(v).to(dev)
+    log(f"generating c2 uint64 column, N=2^{args.log2n}")
+    x = W.c2_uint64_torch(n, device=dev)
     torch.cuda.synchronize()
+    v = x.cpu().numpy()  # the same values for the host partitioner and the oracle
 
     def wall(fn, steps):
         ts = []
@@ -589,10 +589,12 @@ def bench_splitmerge(args):
         else:
             os.environ["GLC_SPLITMERGE_SEGMENT"] = saved
     v6col, v6ts = wall(lambda: glc.compress(x, glc.PART_V6, 0, 2048), max(3, min(args.steps, 20)))
+    log("host split/merge (same library, one call)")
     cpucol, cputs = wall(lambda: glc.compress(x, glc.PART_CPU_SPLITMERGE, 128), 1)
     cpu = None
     if not args.no_cpu_baseline:
         import oracle as O
+        log("oracle exact-sum split/merge on the whole column (one thread)")
         t = time.perf_counter()
         ref = O.compress(v, O.PART_SPLITMERGE_EXACT, 128)
         ref_s = time.perf_counter() - t
@@ -604,7 +606,7 @@ def bench_splitmerge(args):
         "metric": "GPU split/merge compress: raw input GB/s",
         "value": round(n * 8 / (ms * 1e-3) / 1e9, 3), "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "uint64", "data": "synthetic (seeded, generated on the host)",
+        "vs_baseline": None, "dtype": "uint64", "data": "synthetic (seeded, generated on the GPU)",
         "config": {"workload": f"splitmerge_c2_uint64_N2^{args.log2n}_min128", "n": n,
                    "partitions": inf.num_partitions, "compressed_bytes": inf.compressed_size,
                    "timed_total_s": round(t_all, 3)},
@@ -746,7 +748,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=60)
-    ap.add_argument("--log2n", type=int, default=28)
+    ap.add_argument("--log2n", type=int, default=None,
+                    help="column length 2^k (default per workload: c2 28, c3 29, c4 30, splitmerge 24, i32 28)")
     ap.add_argument("--lookups-log2", type=int, default=24)
     ap.add_argument("--mode", default="worksteal", choices=["worksteal", "bitpacked", "v2", "worksteal_adv"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -771,9 +774,9 @@ def main():
               "Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
+    if args.log2n is None:
+        args.log2n = {"c3": 29, "c4": 30, "splitmerge": 24}.get(args.workload, 28)
     if args.workload == "c4":
-        if args.log2n == 28:
-            args.log2n = 30
         return bench_c4(args)
     if args.workload == "c5":
         return bench_c5(args)
@@ -784,12 +787,8 @@ def main():
     if args.workload == "e2e":
         return bench_e2e(args)
     if args.workload == "splitmerge":
-        if args.log2n == 28:
-            args.log2n = 24
         return bench_splitmerge(args)
     if args.workload == "c3":
-        if args.log2n == 28:
-            args.log2n = 29
         return bench_c3(args)
 
     import numpy as np

@@ -242,37 +242,22 @@ __device__ __forceinline__ void wave_narrow(const DecArgs& a, const TileInfo& x,
 // words of the row window (words wb .. wb+128, as wave_narrow's rows).
 // Fewer, wider stores per wave: 8 instead of 16 for 64-bit values, 4
 // instead of 16 for 32-bit values with fields of at most 16 bits.
-// PART = 0: the whole tile; PART = 1: rows h*half .. of the tile's two
-// halves (the half-tile schedule, one wave per half).
-template <class T, int OP, int V, int PART = 0>
-__device__ __forceinline__ void wave_narrow_v(const DecArgs& a, const TileInfo& x, int64_t e0, int64_t e1, int lane,
-                                              int h = 0) {
+// NP = 1: the whole tile; NP = 2: half h of the tile's rows (the half-tile
+// schedule, one wave per half): rows rbase ...
+// FAST: every prediction of the wave's rows lies in dec_fast_range; FULL:
+// every row lies inside the column (no per-row end check).
+template <class T, int OP, int V, bool FAST, bool FULL, int NR>
+__device__ __forceinline__ void narrow_v_rows(const DecArgs& a, const TileInfo& x, int64_t e0, int64_t e1, int lane,
+                                              int rbase, const uint32_t (&x0)[NR], const uint32_t (&x1)[NR],
+                                              const uint32_t (&xs)[NR]) {
     typedef typename std::conditional<OP == OP_UNPACK, int64_t, T>::type O;  // output element
     constexpr int kRowVals = 64 * V;
-    constexpr int kRows = kWaveTile / kRowVals / (PART ? 2 : 1);
-    static_assert(V * sizeof(O) == 16 || V * sizeof(O) == 8, "16- or 8-byte stores");
-    const int rbase = PART ? h * kRows : 0;
     const int b = tile_bits(x.info);
-    const uint32_t* __restrict__ w = a.words;
-    uint32_t x0[kRows], x1[kRows], xs[kRows];
-    // Branch-free loads of the whole 129-word row window, clamped as in
-    // wave_narrow.  The words past the row's fields belong to the next row
-    // or tile; loading only the words each row spans (exec-masked loads)
-    // fetched less but ran 5-10 % slower on every column measured.
-#pragma unroll
-    for (int rr = 0; rr < kRows; ++rr) {
-        const int r = rbase + rr;
-        const int64_t R = x.W + static_cast<int64_t>(r * kRowVals) * b;
-        const int64_t wb = (R >> 5) < a.wlimit ? (R >> 5) : a.wlimit;
-        x0[rr] = w[wb + lane];
-        x1[rr] = w[wb + 64 + lane];
-        xs[rr] = w[wb + 128];
-    }
     const int32_t lbase = x.lbase;
     const uint32_t rel_lane = static_cast<uint32_t>(lane) * static_cast<uint32_t>(V * b);
     O* __restrict__ out = static_cast<O*>(a.out);
 #pragma unroll
-    for (int rr = 0; rr < kRows; ++rr) {
+    for (int rr = 0; rr < NR; ++rr) {
         const int r = rbase + rr;
         const int64_t R = x.W + static_cast<int64_t>(r * kRowVals) * b;
         const uint32_t rel = static_cast<uint32_t>(R & 31) + rel_lane;
@@ -296,11 +281,13 @@ __device__ __forceinline__ void wave_narrow_v(const DecArgs& a, const TileInfo& 
             const int64_t delta = (b == 32) ? static_cast<int64_t>(static_cast<int32_t>(raw))
                                             : static_cast<int64_t>(static_cast<int32_t>(__builtin_amdgcn_sbfe(
                                                   static_cast<int>(raw), 0u, static_cast<unsigned>(b))));
+            const int32_t i = lbase + r * kRowVals + lane * V + k;
             if constexpr (OP == OP_UNPACK) v[k] = delta;
-            else v[k] = apply_delta<T>(dec_pred<T>(x.t0, x.t1, lbase + r * kRowVals + lane * V + k), delta);
+            else if constexpr (FAST) v[k] = apply_delta<T>(dec_pred_fast<T>(x.t0, x.t1, i), delta);
+            else v[k] = apply_delta<T>(dec_pred<T>(x.t0, x.t1, i), delta);
         }
         const int64_t e = e0 + r * kRowVals + lane * V;
-        if (e + V <= e1) {
+        if (FULL || e + V <= e1) {
             if constexpr (V * sizeof(O) == 16) {
                 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
                 u32x4 q;
@@ -329,6 +316,62 @@ __device__ __forceinline__ void wave_narrow_v(const DecArgs& a, const TileInfo& 
                 if (e + k < e1) put(out + e + k, v[k]);
         }
     }
+}
+
+// True when every prediction of local indices i0 .. i1 of the tile's
+// partition lies in dec_fast_range (the same on every lane).
+template <class T>
+__device__ __forceinline__ bool tile_fast(const TileInfo& x, int32_t i0, int32_t i1) {
+    const bool f = dec_fast_range<T>(__builtin_fma(x.t1, static_cast<double>(i0), x.t0),
+                                     __builtin_fma(x.t1, static_cast<double>(i1), x.t0));
+    return __builtin_amdgcn_readfirstlane(static_cast<int>(f)) != 0;
+}
+
+template <class T, int OP, int V, int NP = 1>
+__device__ __forceinline__ void wave_narrow_v(const DecArgs& a, const TileInfo& x, int64_t e0, int64_t e1, int lane,
+                                              int h = 0) {
+    constexpr int kRowVals = 64 * V;
+    constexpr int kRows = kWaveTile / kRowVals / NP;
+    static_assert(kRows >= 1, "a part holds at least one row");
+    static_assert(V * sizeof(typename std::conditional<OP == OP_UNPACK, int64_t, T>::type) == 16 ||
+                      V * sizeof(typename std::conditional<OP == OP_UNPACK, int64_t, T>::type) == 8,
+                  "16- or 8-byte stores");
+    const int rbase = h * kRows;
+    const int b = tile_bits(x.info);
+    const uint32_t* __restrict__ w = a.words;
+    uint32_t x0[kRows], x1[kRows], xs[kRows];
+    // Branch-free loads of the 129-word row window (wb clamped as in
+    // wave_narrow).  Lanes whose word lies past the last word of the wave's
+    // fields (wend) load word wend instead: such words are never selected by
+    // a field, and the wave then touches no cache line of the next wave's
+    // range (which another XCD may be reading) -- 1-2 % faster than the full
+    // window, while exec-masked loads of only the spanned words ran 5-10 %
+    // slower.
+    const int64_t wend = (x.W + static_cast<int64_t>((rbase + kRows) * kRowVals) * b - 1) >> 5;
+#pragma unroll
+    for (int rr = 0; rr < kRows; ++rr) {
+        const int r = rbase + rr;
+        const int64_t R = x.W + static_cast<int64_t>(r * kRowVals) * b;
+        const int64_t wb = (R >> 5) < a.wlimit ? (R >> 5) : a.wlimit;
+        const int64_t d = wend - wb;
+        const uint32_t lim = static_cast<uint32_t>(d < 0 ? 0 : d > 128 ? 128 : d);
+        const uint32_t l0 = static_cast<uint32_t>(lane) < lim ? static_cast<uint32_t>(lane) : lim;
+        const uint32_t l1 = static_cast<uint32_t>(64 + lane) < lim ? static_cast<uint32_t>(64 + lane) : lim;
+        x0[rr] = w[wb + l0];
+        x1[rr] = w[wb + l1];
+        xs[rr] = w[wb + lim];
+    }
+    if constexpr (OP == OP_PACKED) {
+        const int32_t i0 = x.lbase + rbase * kRowVals;
+        if (tile_fast<T>(x, i0, i0 + kRows * kRowVals - 1)) {
+            if (e0 + (rbase + kRows) * kRowVals <= e1)
+                narrow_v_rows<T, OP, V, true, true>(a, x, e0, e1, lane, rbase, x0, x1, xs);
+            else
+                narrow_v_rows<T, OP, V, true, false>(a, x, e0, e1, lane, rbase, x0, x1, xs);
+            return;
+        }
+    }
+    narrow_v_rows<T, OP, V, false, false>(a, x, e0, e1, lane, rbase, x0, x1, xs);
 }
 
 // Wide fast path: one partition, 33..64-bit fields or DIRECT_COPY; two row
@@ -399,33 +442,31 @@ __device__ __forceinline__ void wave_plain(const DecArgs& a, const TileInfo& x, 
 // deltas and one 16-byte (8-byte for 32-bit values) store per lane and row
 // of 128 values.  The delta array is a device allocation (16-byte aligned)
 // and e is even, so the vector loads are aligned; the output's alignment is
-// the caller's (VEC instantiation).
-template <class T>
-__device__ __forceinline__ void wave_plain_v(const DecArgs& a, const TileInfo& x, int64_t e0, int64_t e1, int lane) {
-    constexpr int kRows = kWaveTile / 128;
-    typedef int64_t i64x2 __attribute__((ext_vector_type(2)));
-    i64x2 pl[kRows];
-#pragma unroll
-    for (int r = 0; r < kRows; ++r) {
-        const int64_t e = e0 + r * 128 + 2 * lane;
-        if (e + 2 <= e1) {
-            pl[r] = *reinterpret_cast<const i64x2*>(a.plain + e);
-        } else {
-            pl[r].x = (e < e1) ? a.plain[e] : 0;
-            pl[r].y = 0;
-        }
-    }
-    const int32_t type = tile_type(x.info);
+// the caller's (VEC instantiation).  PRED: 0 = DIRECT_COPY, 1 = dec_pred,
+// 2 = dec_pred_fast.
+typedef int64_t i64x2 __attribute__((ext_vector_type(2)));
+constexpr int kPlainRows = kWaveTile / 128;
+
+template <class T, int PRED>
+__device__ __forceinline__ void plain_v_rows(const DecArgs& a, const TileInfo& x, int64_t e0, int64_t e1, int lane,
+                                             const i64x2 (&pl)[kPlainRows]) {
     const int32_t lbase = x.lbase;
     T* __restrict__ out = static_cast<T*>(a.out);
 #pragma unroll
-    for (int r = 0; r < kRows; ++r) {
+    for (int r = 0; r < kPlainRows; ++r) {
         const int64_t e = e0 + r * 128 + 2 * lane;
         const int32_t i = lbase + r * 128 + 2 * lane;
         T v[2];
-        v[0] = (type == MODEL_DIRECT_COPY) ? static_cast<T>(pl[r].x) : apply_delta<T>(dec_pred<T>(x.t0, x.t1, i), pl[r].x);
-        v[1] = (type == MODEL_DIRECT_COPY) ? static_cast<T>(pl[r].y)
-                                           : apply_delta<T>(dec_pred<T>(x.t0, x.t1, i + 1), pl[r].y);
+        if constexpr (PRED == 0) {
+            v[0] = static_cast<T>(pl[r].x);
+            v[1] = static_cast<T>(pl[r].y);
+        } else if constexpr (PRED == 1) {
+            v[0] = apply_delta<T>(dec_pred<T>(x.t0, x.t1, i), pl[r].x);
+            v[1] = apply_delta<T>(dec_pred<T>(x.t0, x.t1, i + 1), pl[r].y);
+        } else {
+            v[0] = apply_delta<T>(dec_pred_fast<T>(x.t0, x.t1, i), pl[r].x);
+            v[1] = apply_delta<T>(dec_pred_fast<T>(x.t0, x.t1, i + 1), pl[r].y);
+        }
         if (e + 2 <= e1) {
             if constexpr (sizeof(T) == 8) {
                 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
@@ -446,6 +487,24 @@ __device__ __forceinline__ void wave_plain_v(const DecArgs& a, const TileInfo& x
             put(out + e, v[0]);
         }
     }
+}
+
+template <class T>
+__device__ __forceinline__ void wave_plain_v(const DecArgs& a, const TileInfo& x, int64_t e0, int64_t e1, int lane) {
+    i64x2 pl[kPlainRows];
+#pragma unroll
+    for (int r = 0; r < kPlainRows; ++r) {
+        const int64_t e = e0 + r * 128 + 2 * lane;
+        if (e + 2 <= e1) {
+            pl[r] = *reinterpret_cast<const i64x2*>(a.plain + e);
+        } else {
+            pl[r].x = (e < e1) ? a.plain[e] : 0;
+            pl[r].y = 0;
+        }
+    }
+    if (tile_type(x.info) == MODEL_DIRECT_COPY) plain_v_rows<T, 0>(a, x, e0, e1, lane, pl);
+    else if (tile_fast<T>(x, x.lbase, x.lbase + kWaveTile - 1)) plain_v_rows<T, 2>(a, x, e0, e1, lane, pl);
+    else plain_v_rows<T, 1>(a, x, e0, e1, lane, pl);
 }
 
 // Slow path: a tile spanning several partitions, gaps, or a decode-only
@@ -536,7 +595,8 @@ __global__ __launch_bounds__(64 * WPG) void k_decode_wave(DecArgs a) {
 
 // Half tiles (packed decode): one-wave workgroup g decodes half g & 1 of
 // narrow tile g >> 1; a tile of any other path is decoded whole by its
-// first half's wave.
+// first half's wave.  (Quarter tiles, one wave per two 128-value rows, ran
+// 20-55 % slower on every column measured.)
 template <class T, bool VEC>
 __global__ __launch_bounds__(64) void k_decode_half(DecArgs a, int64_t ntiles) {
     const int lane = threadIdx.x & 63;
@@ -550,9 +610,9 @@ __global__ __launch_bounds__(64) void k_decode_half(DecArgs a, int64_t ntiles) {
     }
     const int64_t e0 = t * kWaveTile;
     const int64_t e1 = (e0 + kWaveTile < a.n) ? e0 + kWaveTile : a.n;
-    if constexpr (sizeof(T) == 8) wave_narrow_v<T, OP_PACKED, 2, 1>(a, x, e0, e1, lane, h);
-    else if (tile_bits(x.info) <= 16) wave_narrow_v<T, OP_PACKED, 4, 1>(a, x, e0, e1, lane, h);
-    else wave_narrow_v<T, OP_PACKED, 2, 1>(a, x, e0, e1, lane, h);
+    if constexpr (sizeof(T) == 8) wave_narrow_v<T, OP_PACKED, 2, 2>(a, x, e0, e1, lane, h);
+    else if (tile_bits(x.info) <= 16) wave_narrow_v<T, OP_PACKED, 4, 2>(a, x, e0, e1, lane, h);
+    else wave_narrow_v<T, OP_PACKED, 2, 2>(a, x, e0, e1, lane, h);
 }
 
 // Work stealing.  The tiles form 8 contiguous queues, one per XCD (the
@@ -656,9 +716,9 @@ __device__ __forceinline__ void batch_half(const DecArgs& a, int64_t t, int h, i
     }
     const int64_t e0 = t * kWaveTile;
     const int64_t e1 = (e0 + kWaveTile < a.n) ? e0 + kWaveTile : a.n;
-    if constexpr (sizeof(T) == 8) wave_narrow_v<T, OP_PACKED, 2, 1>(a, x, e0, e1, lane, h);
-    else if (tile_bits(x.info) <= 16) wave_narrow_v<T, OP_PACKED, 4, 1>(a, x, e0, e1, lane, h);
-    else wave_narrow_v<T, OP_PACKED, 2, 1>(a, x, e0, e1, lane, h);
+    if constexpr (sizeof(T) == 8) wave_narrow_v<T, OP_PACKED, 2, 2>(a, x, e0, e1, lane, h);
+    else if (tile_bits(x.info) <= 16) wave_narrow_v<T, OP_PACKED, 4, 2>(a, x, e0, e1, lane, h);
+    else wave_narrow_v<T, OP_PACKED, 2, 2>(a, x, e0, e1, lane, h);
 }
 
 __global__ __launch_bounds__(64) void k_batch_decode_half(const BatchCol* __restrict__ cols,
@@ -738,7 +798,7 @@ DecArgs make_args(const glc_column* c, void* out) {
 template <class T, int OP, bool VEC> void launch_static(const DecArgs& a, int dsched, int64_t nwt, hipStream_t s) {
     if (dsched == GLC_DECODE_HALF_TILE_W1 && OP == OP_PACKED)
         hipLaunchKernelGGL((k_decode_half<T, VEC>), dim3(static_cast<unsigned>(2 * nwt)), dim3(64), 0, s, a, nwt);
-    else if (dsched == GLC_DECODE_TILE_W1 || dsched == GLC_DECODE_HALF_TILE_W1)
+    else if (dsched != GLC_DECODE_TILE_W4)
         hipLaunchKernelGGL((k_decode_wave<T, OP, VEC, 1>), dim3(static_cast<unsigned>(nwt)), dim3(64), 0, s, a);
     else
         hipLaunchKernelGGL((k_decode_wave<T, OP, VEC, 4>), dim3(static_cast<unsigned>((nwt + 3) / 4)), dim3(256), 0, s,

@@ -346,6 +346,63 @@ def test_parity_decode_only_polynomial_models(gpu):
     assert np.array_equal(host(h.random_access_gpu(to_dev(pos))), want[pos])
 
 
+def crafted_models(dt):
+    """(theta0, theta1) per 4096-value partition: ties, runs that cross the
+    decoders' fast-range bounds (2^50 for 64-bit values, the 32-bit limits)
+    inside a partition, unsigned runs that start below zero, NaN."""
+    width = 64 if dt in (glc.I64, glc.U64) else 32
+    signed = dt in (glc.I32, glc.I64)
+    hi = float(2 ** 50) if width == 64 else (2147483646.0 if signed else 4294967294.0)
+    C = 0.49999999999999994
+    m = [(0.5, 1.0), (-0.5, -1.0), (C, 0.0), (0.5, 0.0), (2.5, 2.0 ** -53), (-1000.5, 0.5),
+         (hi - 2048.5, 1.0), (hi - 1000.25, 0.5), (hi, 0.0), (hi - 0.5, 0.0), (hi + 10.0, 0.0),
+         (hi - 3 * 4095.0, 3.0), (float("nan"), 1.0), (1.0, float("inf")), (123456.75, 0.125), (7.0, -0.5)]
+    if signed:
+        m += [(-hi + 2048.5, -1.0), (-hi, 0.0), (-hi - 0.5, 0.0), (-hi + 1.0, -1.0 / 4095)]
+    if width == 64:
+        m += [(2.0 ** 51 - 100.0, 0.0), (2.0 ** 50 + 0.25, 0.0), (2.0 ** 50 - 0.25, 0.0)]
+    return m
+
+
+@pytest.mark.parametrize("dt,noise", [(glc.I32, 8), (glc.I32, 24), (glc.U32, 12), (glc.U32, 30),
+                                      (glc.I64, 10), (glc.I64, 30), (glc.U64, 6), (glc.U64, 20)])
+def test_parity_crafted_predictions(gpu, dt, noise):
+    # the decoders convert predictions without saturation branches when a
+    # wave's run of indices lies in the fast range; crafted parameters put
+    # whole tiles, half tiles and rows on both sides of its bounds, and every
+    # decoder must equal the oracle's decode rule on them
+    npdt = {glc.I32: np.int32, glc.U32: np.uint32, glc.I64: np.int64, glc.U64: np.uint64}[dt]
+    models = crafted_models(dt)
+    n = 4096 * len(models) - 777  # ragged last partition
+    rng = np.random.default_rng(noise)
+    v = rng.integers(0, 2 ** noise, n).astype(npdt)
+    ref = O.compress(v, O.PART_FIXED, 4096)
+    assert np.all(ref.model_type == O.MODEL_LINEAR) and ref.bits.max() <= 32
+    b = bytearray(ref.blob)
+    hdr = struct.unpack_from("<4I10Q", b, 0)
+    params_off = hdr[7]
+    for p, (t0, t1) in enumerate(models):
+        struct.pack_into("<dd", b, params_off + 32 * p, t0, t1)
+        ref.params[p, 0], ref.params[p, 1] = t0, t1
+    blob = _reseal(b)
+    want = O.decode_full(ref)
+    d = glc.deserialize(blob, dt)
+    for sch in (glc.DECODE_TILE_W1, glc.DECODE_TILE_W4, glc.DECODE_HALF_TILE_W1):
+        d.set_decode_schedule(sch)
+        for mode in (glc.FULL_AUTO, glc.FULL_BITPACKED, glc.FULL_WORKSTEAL):
+            assert np.array_equal(host(d.decompress_full(mode)), want), (sch, mode)
+    for mode in (glc.FULL_V2, glc.FULL_WORKSTEAL_ADV, 102):
+        assert np.array_equal(host(d.decompress_full(mode)), want), mode
+    buf = torch.empty(n + 1, dtype=to_dev(v[:1]).dtype, device="cuda")
+    d.decompress_full(glc.FULL_BITPACKED, out=buf[1:])  # unaligned output
+    assert np.array_equal(host(buf[1:]), want)
+    pos = np.random.default_rng(3).integers(0, n, 5000).astype(np.int32)
+    assert np.array_equal(host(d.random_access(to_dev(pos))), want[pos])
+    d.preunpack()
+    assert np.array_equal(host(d.decompress_full(glc.FULL_PREUNPACKED)), want)
+    assert np.array_equal(host(d.decompress_full(glc.FULL_AUTO)), want)
+
+
 def test_empty_column_and_errors(gpu):
     col = glc.compress(to_dev(np.zeros(0, np.int64)))
     assert col.n == 0 and col.num_partitions == 0

@@ -31,6 +31,18 @@ def decoders(col, x, q=1 << 22):
         col.decompress_full(mode, out=out)
         torch.cuda.synchronize()
         assert torch.equal(out.view(torch.int64), x.view(torch.int64)), mode
+    # every packed decode schedule at full size, then the one the tuner keeps
+    for sch in (glc.DECODE_TILE_W1, glc.DECODE_HALF_TILE_W1, glc.DECODE_TILE_W4):
+        col.set_decode_schedule(sch)
+        out.zero_()
+        col.decompress_full(glc.FULL_BITPACKED, out=out)
+        torch.cuda.synchronize()
+        assert torch.equal(out.view(torch.int64), x.view(torch.int64)), sch
+    col.autotune_decode(out, reps=2)
+    out.zero_()
+    col.decompress_full(glc.FULL_WORKSTEAL, out=out)
+    torch.cuda.synchronize()
+    assert torch.equal(out.view(torch.int64), x.view(torch.int64)), "tuned"
     pos = torch.from_numpy(W.positions_mixed(q, x.numel())).cuda()
     r = col.random_access(pos)
     torch.cuda.synchronize()
