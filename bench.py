@@ -36,6 +36,24 @@ META_BYTES_PER_PARTITION = 36
 LOOKUP_LINE_BYTES = 4 + 8 + 64 + 64
 SCHEDULE_NAMES = {0: "tile per one-wave workgroup", 1: "4 tiles per four-wave workgroup",
                   2: "half tile per one-wave workgroup"}
+# the kernel each decode schedule launches (packed column, 16-byte aligned output)
+SCHEDULE_KERNEL = {0: "k_decode_wave<{T}, packed, 1 wave/workgroup>", 1: "k_decode_wave<{T}, packed, 4 waves/workgroup>",
+                   2: "k_decode_half<{T}>"}
+
+
+def decode_traffic(workload: str, sched: int):
+    """HBM bytes per decode launch of `workload` with schedule `sched`, from
+    profiles/decode_traffic.json: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
+    of that kernel, scaled by the same counters on a plain 2 GiB copy, kept
+    with the commit they were measured on.  (None, None) if not measured."""
+    tf = os.path.join(ROOT, "profiles", "decode_traffic.json")
+    try:
+        tj = json.load(open(tf))
+        per = tj["workloads"][workload]["per_schedule"][str(sched)]
+        return per["bytes"], (f"profiles/decode_traffic.json ({workload}, {SCHEDULE_NAMES[sched]}: rocprofv3 --pmc "
+                              f"at commit {tj.get('commit', '?')})")
+    except (OSError, KeyError, TypeError, ValueError):
+        return None, None
 # bench --mode -> the decode kernel that mode launches
 DECODER_KERNEL = {
     "worksteal": "k_decode_wave (GLC_FULL_WORKSTEAL: one wave per 1024-value tile, tiles handed to CUs by the "
@@ -297,8 +315,9 @@ def bench_i32(args):
                    "partitions": inf.num_partitions, "compressed_bytes": inf.compressed_size,
                    "decode_schedule": SCHEDULE_NAMES[sched]},
         "roofline": {"bound": "hbm", "achieved": round(alg / (ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_decode_wave (int32, bit-packed)", "algorithmic_bytes": alg},
+                     "unit": "GB/s", "frac": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "traffic": decode_traffic("i32", sched)[0], "traffic_source": decode_traffic("i32", sched)[1],
+                     "kernel": SCHEDULE_KERNEL[sched].format(T="int32"), "algorithmic_bytes": alg},
         "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
@@ -375,7 +394,8 @@ def bench_c3(args):
         "roofline": {"bound": "hbm", "achieved": round(alg_packed / (res["packed_full_ms"] * 1e-3) / 1e9, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(alg_packed / (res["packed_full_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                     "traffic": None, "kernel": "k_decode_wave (int64, bit-packed)", "algorithmic_bytes": alg_packed,
+                     "traffic": decode_traffic("c3", sched)[0], "traffic_source": decode_traffic("c3", sched)[1],
+                     "kernel": SCHEDULE_KERNEL[sched].format(T="int64"), "algorithmic_bytes": alg_packed,
                      "preunpacked_frac": round(alg_plain / (res["plain_full_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
         "cpu_baseline": None,
     }
@@ -894,15 +914,7 @@ def main():
     # FETCH_SIZE / WRITE_SIZE per decode launch, measured by two rocprofv3
     # --pmc runs of this program (scaled by the same counters on a plain copy)
     # and kept with the commit they were measured on
-    traffic, traffic_src = None, None
-    tf = os.path.join(ROOT, "profiles", "decode_traffic.json")
-    if os.path.exists(tf):
-        try:
-            tj = json.load(open(tf))
-            traffic = tj.get("bytes_per_launch")
-            traffic_src = f"profiles/decode_traffic.json (rocprofv3 --pmc, measured at commit {tj.get('commit', '?')})"
-        except Exception:
-            traffic = None
+    traffic, traffic_src = decode_traffic("c2", sched)
     look_traffic = None
     lf = os.path.join(ROOT, "profiles", "lookup_traffic.json")
     if os.path.exists(lf):
@@ -951,7 +963,7 @@ def main():
         "compress_ms": round(compress_ms_steady, 2), "compress_ms_first_call": round(compress_ms, 2),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_src,
-                     "kernel": "k_decode_wave<uint64, packed>", "algorithmic_bytes": alg_bytes},
+                     "kernel": SCHEDULE_KERNEL[sched].format(T="uint64"), "algorithmic_bytes": alg_bytes},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -258,46 +258,143 @@ __device__ __forceinline__ void fit_wave(const T* __restrict__ x, int32_t n, int
     }
 }
 
+// Partition p of 0 < n <= 512 * NSTEP values, fitted by one wave that holds
+// the partition in registers: lane l keeps values i = 64 j + l (j < 8 NSTEP),
+// which are exactly fit_wave's loads, summed in fit_wave's order (set j & 3,
+// increasing i within a set); the max-error pass reads the same registers,
+// so the partition crosses HBM once.
+// A zero the compiler cannot see through.  Index values that depend only on
+// the lane (i, (double)i, i*i) are loop invariants of a loop over
+// partitions; hoisted out of it they would stay live in registers for the
+// whole loop, and the max-error pass would keep the sums pass's (double)i
+// of every value (256 VGPRs, occupancy 1 for 64-bit values; 160 and 3 with
+// the lane taken afresh from this zero in the loop and again in each pass).
+__device__ __forceinline__ int32_t opaque_zero() {
+    int32_t z;
+    asm volatile("v_mov_b32 %0, 0" : "=v"(z));
+    return z;
+}
+
+template <class T, int FORM, int NSTEP>
+__device__ __forceinline__ void fit_wave_reg(const T* __restrict__ x, int32_t n, int64_t p,
+                                             int32_t* __restrict__ type, double* __restrict__ params,
+                                             int32_t* __restrict__ bits, int64_t* __restrict__ err, int lane_) {
+    constexpr int J = 8 * NSTEP;
+    const int lane = lane_ + opaque_zero();
+    T y[J];
+#pragma unroll
+    for (int j = 0; j < J; ++j) {  // branch-free: every load in flight at once
+        const int32_t i = 64 * j + lane;
+        y[j] = x[i < n ? i : n - 1];
+    }
+    double a[4] = {0.0, 0.0, 0.0, 0.0}, b[4] = {0.0, 0.0, 0.0, 0.0};
+    double c[4] = {0.0, 0.0, 0.0, 0.0}, d[4] = {0.0, 0.0, 0.0, 0.0};
+    int ovf = 0;
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        const int32_t i = 64 * j + lane;
+        if (i < n) {
+            ovf |= might_overflow_double<T>(y[j]) ? 1 : 0;
+            const double xi = static_cast<double>(i);
+            const double yi = static_cast<double>(y[j]);
+            a[j & 3] = a[j & 3] + xi;
+            b[j & 3] = b[j & 3] + yi;
+            c[j & 3] = acc_prod<FORM>(xi, xi, c[j & 3]);
+            d[j & 3] = acc_prod<FORM>(xi, yi, d[j & 3]);
+        }
+    }
+    if (__any(ovf)) {  // :95-106 DIRECT_COPY
+        if (lane == 0) {
+            type[p] = MODEL_DIRECT_COPY;
+            params[4 * p] = 0.0; params[4 * p + 1] = 0.0; params[4 * p + 2] = 0.0; params[4 * p + 3] = 0.0;
+            err[p] = 0;
+            bits[p] = static_cast<int32_t>(sizeof(T) * 8);
+        }
+        return;
+    }
+    double Sx = (a[0] + a[2]) + (a[1] + a[3]);
+    double Sy = (b[0] + b[2]) + (b[1] + b[3]);
+    double Sxx = (c[0] + c[2]) + (c[1] + c[3]);
+    double Sxy = (d[0] + d[2]) + (d[1] + d[3]);
+    for (int h = 32; h > 0; h >>= 1) {
+        Sx = Sx + __shfl_down(Sx, h, 64);
+        Sy = Sy + __shfl_down(Sy, h, 64);
+        Sxx = Sxx + __shfl_down(Sxx, h, 64);
+        Sxy = Sxy + __shfl_down(Sxy, h, 64);
+    }
+    Sx = __shfl(Sx, 0, 64);
+    Sy = __shfl(Sy, 0, 64);
+    Sxx = __shfl(Sxx, 0, 64);
+    Sxy = __shfl(Sxy, 0, 64);
+    double t0, t1;
+    fit_theta<FORM>(n, Sx, Sy, Sxx, Sxy, t0, t1);
+    int64_t mx = 0;
+    const int lane2 = lane_ + opaque_zero();  // (double)i afresh: not kept live from the sums pass
+    // the clamped index repeats element n - 1: same value
+    if (enc_fast_range<T>(__builtin_fma(t1, 0.0, t0), __builtin_fma(t1, static_cast<double>(n - 1), t0))) {
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+            const int32_t i = 64 * j + lane2;
+            const int64_t dl = enc_delta_fast<T>(y[j], t0, t1, i < n ? i : n - 1);
+            const int64_t ab = dl < 0 ? static_cast<int64_t>(0ULL - static_cast<uint64_t>(dl)) : dl;
+            mx = ab > mx ? ab : mx;
+        }
+    } else {  // predictions outside the fast range (rare): read the partition again, in a short loop
+        for (int32_t i = lane2; i < n; i += 64) {
+            const int64_t ab = abs_delta<T>(x[i], t0, t1, i);
+            mx = ab > mx ? ab : mx;
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const int64_t q = __shfl_xor(mx, o, 64);
+        mx = q > mx ? q : mx;
+    }
+    if (lane == 0) {
+        type[p] = MODEL_LINEAR;
+        params[4 * p] = t0; params[4 * p + 1] = t1; params[4 * p + 2] = 0.0; params[4 * p + 3] = 0.0;
+        err[p] = mx;
+        bits[p] = bits_for(mx);
+    }
+}
+
 // Partitions of kFitRegMin..kFitRegMax values (block size 256): one
 // 256-thread workgroup per partition, i.e. the reference's block itself.
-// Thread t loads its values i = t, t+256, ... (at most kFitRegK of them) into
+// Thread t loads its values i = t, t+256, ... (at most K of them) into
 // registers once, sums them in increasing i, and the tree s = 128, 64 runs
 // through LDS, s = 32..1 by shuffles in wave 0; the max-error pass then reads
 // the same registers, so the partition crosses HBM once.  Measured on int64
 // 2^29 with 4096-value partitions: 0.76 ms against 1.10 ms for the wave form.
-// Smaller partitions stay with the wave form (a workgroup per 1024 values
-// spends more on its barriers than the second read costs), and so do larger
-// ones (32 values per thread in registers cost occupancy).
+// Smaller partitions stay with the wave forms (a workgroup per 1024 values
+// spends more on its barriers than the second read costs).
 constexpr int kFitRegK = 16;
 constexpr int32_t kFitRegMin = 2048;
 constexpr int32_t kFitRegMax = 256 * kFitRegK;
+constexpr int32_t kFitRegMax32 = 256 * 32;  // the k_fit_chunk form's largest partition
+constexpr int32_t kFitWaveRegMax = 2048;    // fit_wave_reg<.., 4>'s largest partition
 
-template <class T, int FORM>
-__global__ __launch_bounds__(256) void k_fit_reg(const T* __restrict__ v, const int32_t* __restrict__ st,
-                                                 const int32_t* __restrict__ en, int32_t* __restrict__ type,
-                                                 double* __restrict__ params, int32_t* __restrict__ bits,
-                                                 int64_t* __restrict__ err, int64_t P) {
-    __shared__ double s_sum[4][128];
-    __shared__ double s_th[2];
-    __shared__ int64_t s_mx[4];
-    const int tid = threadIdx.x;
+struct FitRegLds {
+    double sum[4][128];
+    double th[2];
+    int64_t mx[4];
+};
+
+// Called by the whole 256-thread workgroup (uniform n, 0 < n <= 256 K).
+template <class T, int FORM, int K>
+__device__ __forceinline__ void fit_block_reg(const T* __restrict__ x, int32_t n, int64_t p,
+                                              int32_t* __restrict__ type, double* __restrict__ params,
+                                              int32_t* __restrict__ bits, int64_t* __restrict__ err, FitRegLds& L) {
+    const int tid = static_cast<int>(threadIdx.x) + opaque_zero();
     const int lane = tid & 63, wave = tid >> 6;
-    const int64_t p = blockIdx.x;
-    if (p >= P) return;
-    const int32_t s0 = st[p];
-    const int32_t n = en[p] - s0;
-    if (n < kFitRegMin || n > kFitRegMax) return;  // the wave form's partitions (uniform per workgroup)
-    const T* __restrict__ x = v + s0;
-    T y[kFitRegK];
+    T y[K];
 #pragma unroll
-    for (int k = 0; k < kFitRegK; ++k) {  // branch-free: every load in flight at once
+    for (int k = 0; k < K; ++k) {  // branch-free: every load in flight at once
         const int32_t i = tid + 256 * k;
         y[k] = x[i < n ? i : n - 1];
     }
     double a = 0.0, b = 0.0, c = 0.0, d = 0.0;
     int ovf = 0;
 #pragma unroll
-    for (int k = 0; k < kFitRegK; ++k) {  // CompressionKernels.cuh:113-120, thread t's own order
+    for (int k = 0; k < K; ++k) {  // CompressionKernels.cuh:113-120, thread t's own order
         const int32_t i = tid + 256 * k;
         if (i < n) {
             ovf |= might_overflow_double<T>(y[k]) ? 1 : 0;
@@ -320,19 +417,19 @@ __global__ __launch_bounds__(256) void k_fit_reg(const T* __restrict__ v, const 
     }
     // tree (:129-137): s = 128 (t < 128 adds t + 128), s = 64, then 32..1 in wave 0
     if (tid >= 128) {
-        s_sum[0][tid - 128] = a; s_sum[1][tid - 128] = b; s_sum[2][tid - 128] = c; s_sum[3][tid - 128] = d;
+        L.sum[0][tid - 128] = a; L.sum[1][tid - 128] = b; L.sum[2][tid - 128] = c; L.sum[3][tid - 128] = d;
     }
     __syncthreads();
     if (tid < 128) {
-        a = a + s_sum[0][tid]; b = b + s_sum[1][tid]; c = c + s_sum[2][tid]; d = d + s_sum[3][tid];
+        a = a + L.sum[0][tid]; b = b + L.sum[1][tid]; c = c + L.sum[2][tid]; d = d + L.sum[3][tid];
     }
     __syncthreads();
     if (tid >= 64 && tid < 128) {
-        s_sum[0][tid - 64] = a; s_sum[1][tid - 64] = b; s_sum[2][tid - 64] = c; s_sum[3][tid - 64] = d;
+        L.sum[0][tid - 64] = a; L.sum[1][tid - 64] = b; L.sum[2][tid - 64] = c; L.sum[3][tid - 64] = d;
     }
     __syncthreads();
     if (wave == 0) {
-        a = a + s_sum[0][tid]; b = b + s_sum[1][tid]; c = c + s_sum[2][tid]; d = d + s_sum[3][tid];
+        a = a + L.sum[0][tid]; b = b + L.sum[1][tid]; c = c + L.sum[2][tid]; d = d + L.sum[3][tid];
         for (int h = 32; h > 0; h >>= 1) {
             a = a + __shfl_down(a, h, 64);
             b = b + __shfl_down(b, h, 64);
@@ -342,27 +439,26 @@ __global__ __launch_bounds__(256) void k_fit_reg(const T* __restrict__ v, const 
         if (lane == 0) {
             double t0, t1;
             fit_theta<FORM>(n, a, b, c, d, t0, t1);
-            s_th[0] = t0;
-            s_th[1] = t1;
+            L.th[0] = t0;
+            L.th[1] = t1;
         }
     }
     __syncthreads();
-    const double t0 = s_th[0], t1 = s_th[1];
+    const double t0 = L.th[0], t1 = L.th[1];
     int64_t mx = 0;
+    const int tid2 = static_cast<int>(threadIdx.x) + opaque_zero();  // (double)i afresh, as in fit_wave_reg
     // the clamped index repeats element n - 1: same value
     if (enc_fast_range<T>(__builtin_fma(t1, 0.0, t0), __builtin_fma(t1, static_cast<double>(n - 1), t0))) {
 #pragma unroll
-        for (int k = 0; k < kFitRegK; ++k) {
-            const int32_t i = tid + 256 * k;
+        for (int k = 0; k < K; ++k) {
+            const int32_t i = tid2 + 256 * k;
             const int64_t dl = enc_delta_fast<T>(y[k], t0, t1, i < n ? i : n - 1);
             const int64_t ab = dl < 0 ? static_cast<int64_t>(0ULL - static_cast<uint64_t>(dl)) : dl;
             mx = ab > mx ? ab : mx;
         }
-    } else {
-#pragma unroll
-        for (int k = 0; k < kFitRegK; ++k) {
-            const int32_t i = tid + 256 * k;
-            const int64_t ab = abs_delta<T>(y[k], t0, t1, i < n ? i : n - 1);
+    } else {  // predictions outside the fast range (rare): read the partition again, in a short loop
+        for (int32_t i = tid2; i < n; i += 256) {
+            const int64_t ab = abs_delta<T>(x[i], t0, t1, i);
             mx = ab > mx ? ab : mx;
         }
     }
@@ -370,16 +466,31 @@ __global__ __launch_bounds__(256) void k_fit_reg(const T* __restrict__ v, const 
         const int64_t q = __shfl_xor(mx, o, 64);
         mx = q > mx ? q : mx;
     }
-    if (lane == 0) s_mx[wave] = mx;
+    if (lane == 0) L.mx[wave] = mx;
     __syncthreads();
     if (tid == 0) {
-        int64_t m = s_mx[0];
-        for (int w = 1; w < 4; ++w) m = s_mx[w] > m ? s_mx[w] : m;
+        int64_t m = L.mx[0];
+        for (int w = 1; w < 4; ++w) m = L.mx[w] > m ? L.mx[w] : m;
         type[p] = MODEL_LINEAR;
         params[4 * p] = t0; params[4 * p + 1] = t1; params[4 * p + 2] = 0.0; params[4 * p + 3] = 0.0;
         err[p] = m;
         bits[p] = bits_for(m);
     }
+    __syncthreads();  // L is reused by the workgroup's next partition
+}
+
+template <class T, int FORM>
+__global__ __launch_bounds__(256) void k_fit_reg(const T* __restrict__ v, const int32_t* __restrict__ st,
+                                                 const int32_t* __restrict__ en, int32_t* __restrict__ type,
+                                                 double* __restrict__ params, int32_t* __restrict__ bits,
+                                                 int64_t* __restrict__ err, int64_t P) {
+    __shared__ FitRegLds L;
+    const int64_t p = blockIdx.x;
+    if (p >= P) return;
+    const int32_t s0 = st[p];
+    const int32_t n = en[p] - s0;
+    if (n < kFitRegMin || n > kFitRegMax) return;  // the wave form's partitions (uniform per workgroup)
+    fit_block_reg<T, FORM, kFitRegK>(v + s0, n, p, type, params, bits, err, L);
 }
 
 template <class T, int FORM>
@@ -395,6 +506,43 @@ __global__ __launch_bounds__(64 * kFitWaves) void k_fit_wave(const T* __restrict
     if (n <= 0) return;  // :62, entry keeps its initial values
     if (skip_reg && n >= kFitRegMin && n <= kFitRegMax) return;  // k_fit_reg's partitions
     fit_wave<T, FORM>(v + s0, n, p, type, params, bits, err, lane);
+}
+
+// Mixed partition sizes (V6, split/merge, other fixed sizes), block size
+// 256, every partition read from HBM once when it has at most kFitRegMax32
+// values: workgroup g takes partitions g*kFitChunk .. + kFitChunk.  Its waves
+// first fit the partitions of at most kFitWaveRegMax values from registers
+// (one wave each, round robin) and those above kFitRegMax32 with the
+// two-pass wave form; then the whole workgroup fits the ones in between, one
+// after the other, from registers (fit_block_reg).  Results do not depend on
+// who fits a partition: each one follows the reference's order on its own.
+constexpr int kFitChunk = 16;
+
+template <class T, int FORM>
+__global__ __launch_bounds__(256) void k_fit_chunk(const T* __restrict__ v, const int32_t* __restrict__ st,
+                                                   const int32_t* __restrict__ en, int32_t* __restrict__ type,
+                                                   double* __restrict__ params, int32_t* __restrict__ bits,
+                                                   int64_t* __restrict__ err, int64_t P) {
+    __shared__ FitRegLds L;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t p0 = static_cast<int64_t>(blockIdx.x) * kFitChunk;
+    const int64_t p1 = (p0 + kFitChunk < P) ? p0 + kFitChunk : P;
+    for (int64_t p = p0 + wave; p < p1; p += 4) {
+        const int32_t s0 = st[p];
+        const int32_t n = en[p] - s0;
+        if (n <= 0) continue;  // :62, entry keeps its initial values
+        if (n <= 1024) fit_wave_reg<T, FORM, 2>(v + s0, n, p, type, params, bits, err, lane);
+        else if (n <= kFitWaveRegMax) fit_wave_reg<T, FORM, 4>(v + s0, n, p, type, params, bits, err, lane);
+        else if (n > kFitRegMax32) fit_wave<T, FORM>(v + s0, n, p, type, params, bits, err, lane);
+    }
+    for (int64_t p = p0; p < p1; ++p) {  // uniform over the workgroup
+        const int32_t s0 = st[p];
+        const int32_t n = en[p] - s0;
+        if (n <= kFitWaveRegMax || n > kFitRegMax32) continue;
+        if (n <= kFitRegMax) fit_block_reg<T, FORM, kFitRegK>(v + s0, n, p, type, params, bits, err, L);
+        else fit_block_reg<T, FORM, 32>(v + s0, n, p, type, params, bits, err, L);
+    }
 }
 
 template <class T, int FORM>
@@ -1284,19 +1432,24 @@ int launch_fit_t(const void* d_values, glc_column* c, int32_t bs, int32_t fixed_
     if (bs == kFitWaveBs) {
         // fixed partitions of kFitRegMin..kFitRegMax values: the register
         // form (and k_fit_wave's launch covers only the last, shorter one);
-        // mixed sizes (V6, split/merge): the wave form alone -- two launches
-        // each skipping the other's partitions ran 602 us on the uint64 2^28
-        // V6 column against 555 us for the wave form over all of them
+        // every other layout: k_fit_chunk (one launch; a second launch that
+        // skips the first one's partitions costs more than it saves: 602 us
+        // against 555 us on the uint64 2^28 V6 column)
         if (fixed_size >= kFitRegMin && fixed_size <= kFitRegMax) {
             hipLaunchKernelGGL((k_fit_reg<T, FORM>), dim3(static_cast<unsigned>(c->P)), dim3(256), 0, s,
                                static_cast<const T*>(d_values), c->d_start, c->d_end, c->d_type, c->d_params,
                                c->d_bits, c->d_err, c->P);
             GLC_HIP(hipGetLastError());
+            const unsigned g = static_cast<unsigned>((c->P + kFitWaves - 1) / kFitWaves);
+            hipLaunchKernelGGL((k_fit_wave<T, FORM>), dim3(g), dim3(64 * kFitWaves), 0, s,
+                               static_cast<const T*>(d_values), c->d_start, c->d_end, c->d_type, c->d_params,
+                               c->d_bits, c->d_err, c->P, true);
+            GLC_HIP(hipGetLastError());
+            return GLC_OK;
         }
-        const unsigned g = static_cast<unsigned>((c->P + kFitWaves - 1) / kFitWaves);
-        hipLaunchKernelGGL((k_fit_wave<T, FORM>), dim3(g), dim3(64 * kFitWaves), 0, s, static_cast<const T*>(d_values),
-                           c->d_start, c->d_end, c->d_type, c->d_params, c->d_bits, c->d_err, c->P,
-                           fixed_size >= kFitRegMin && fixed_size <= kFitRegMax);
+        const unsigned g = static_cast<unsigned>((c->P + kFitChunk - 1) / kFitChunk);
+        hipLaunchKernelGGL((k_fit_chunk<T, FORM>), dim3(g), dim3(256), 0, s, static_cast<const T*>(d_values),
+                           c->d_start, c->d_end, c->d_type, c->d_params, c->d_bits, c->d_err, c->P);
         GLC_HIP(hipGetLastError());
         return GLC_OK;
     }
