@@ -55,10 +55,10 @@ def decode_traffic(workload: str, sched: int):
     except (OSError, KeyError, TypeError, ValueError):
         return None, None
 # bench --mode -> the decode kernel that mode launches
+# ("{K}": the kernel of the column's decode schedule)
 DECODER_KERNEL = {
-    "worksteal": "k_decode_wave (GLC_FULL_WORKSTEAL: one wave per 1024-value tile, tiles handed to CUs by the "
-                 "hardware workgroup dispatcher)",
-    "bitpacked": "k_decode_wave (GLC_FULL_BITPACKED)",
+    "worksteal": "{K} (GLC_FULL_WORKSTEAL: wave tiles handed to CUs by the hardware workgroup dispatcher)",
+    "bitpacked": "{K} (GLC_FULL_BITPACKED)",
     "v2": "k_decode_parts<persistent=false> (GLC_FULL_V2: one workgroup per partition)",
     "worksteal_adv": "k_decode_parts<persistent=true> (GLC_FULL_WORKSTEAL_ADV: workgroups claim partitions from a counter)",
 }
@@ -914,7 +914,7 @@ def main():
     # FETCH_SIZE / WRITE_SIZE per decode launch, measured by two rocprofv3
     # --pmc runs of this program (scaled by the same counters on a plain copy)
     # and kept with the commit they were measured on
-    traffic, traffic_src = decode_traffic("c2", sched)
+    traffic, traffic_src = decode_traffic("c2", sched) if args.mode in ("worksteal", "bitpacked") else (None, None)
     look_traffic = None
     lf = os.path.join(ROOT, "profiles", "lookup_traffic.json")
     if os.path.exists(lf):
@@ -942,7 +942,8 @@ def main():
         "dtype": "u64",
         "data": "synthetic (seeded generator of the configured shape, generated on the GPU)",
         "config": {"workload": f"c2_uint64_piecewise_linear_N2^{args.log2n}_v6", "n": n, "partitioner": "v6",
-                   "v6_base": 2048, "decoder": DECODER_KERNEL[args.mode], "decode_schedule": SCHEDULE_NAMES[sched],
+                   "v6_base": 2048, "decoder": DECODER_KERNEL[args.mode].format(K=SCHEDULE_KERNEL[sched].format(T="uint64")),
+                   "decode_schedule": SCHEDULE_NAMES[sched],
                    "lookups_per_step": q,
                    "lookup_mix": "1/2 uniform + 1/2 zipf(1.1) scrambled", "partitions": inf.num_partitions,
                    "compressed_bytes": inf.compressed_size, "compression_ratio": round(n * 8 / inf.compressed_size, 4)},
@@ -963,7 +964,9 @@ def main():
         "compress_ms": round(compress_ms_steady, 2), "compress_ms_first_call": round(compress_ms, 2),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_src,
-                     "kernel": SCHEDULE_KERNEL[sched].format(T="uint64"), "algorithmic_bytes": alg_bytes},
+                     "kernel": (SCHEDULE_KERNEL[sched].format(T="uint64") if args.mode in ("worksteal", "bitpacked")
+                                else DECODER_KERNEL[args.mode].split(" (")[0]),
+                     "algorithmic_bytes": alg_bytes},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
