@@ -88,8 +88,16 @@ template <class T> __device__ __forceinline__ bool enc_fast_range(double x_lo, d
     else return lo >= 0.0 && hi <= 1125899906842624.0;
 }
 
+// round(p), half away from zero, for every double p: the sum reaches the next
+// integer exactly when p's fraction is at least 1/2 (0.5 - 2^-54 is the
+// largest double below 1/2).  Three instructions against six for the
+// library round.
+__device__ __forceinline__ double round_away(double p) {
+    return __builtin_trunc(p + __builtin_copysign(0.49999999999999994, p));
+}
+
 template <class T> __device__ __forceinline__ int64_t enc_delta_fast(T a, double t0, double t1, int32_t i) {
-    const double r = __builtin_round(__builtin_fma(t1, static_cast<double>(i), t0));
+    const double r = round_away(__builtin_fma(t1, static_cast<double>(i), t0));
     if constexpr (sizeof(T) == 4) {
         return static_cast<int64_t>(a) - static_cast<int64_t>(static_cast<T>(r));
     } else {
@@ -113,7 +121,7 @@ template <class T> __device__ __forceinline__ bool might_overflow_double(T v) {
 
 // Encoder prediction: (T)round(fma(theta1, i, theta0)) -- CompressionKernels.cuh:168-169, :318-324.
 template <class T> __device__ __forceinline__ T enc_pred(double t0, double t1, int32_t i) {
-    return sat_cast<T>(__builtin_round(__builtin_fma(t1, static_cast<double>(i), t0)));
+    return sat_cast<T>(round_away(__builtin_fma(t1, static_cast<double>(i), t0)));
 }
 
 // Decoder prediction with the decoders' clamps (FullDecompressionKernels.cuh:157-189).
@@ -123,7 +131,7 @@ template <class T> __device__ __forceinline__ T dec_pred(double t0, double t1, i
         p = __builtin_fmax(0.0, p);
         if constexpr (sizeof(T) == 4) p = __builtin_fmin(p, 4294967295.0);
     }
-    return sat_cast<T>(__builtin_round(p));
+    return sat_cast<T>(round_away(p));
 }
 
 // Decoder prediction without branches, for a run of indices whose
@@ -150,12 +158,10 @@ template <class T> __device__ __forceinline__ bool dec_fast_range(double p_lo, d
 }
 
 template <class T> __device__ __forceinline__ T dec_pred_fast(double t0, double t1, int32_t i) {
-    constexpr double kBelowHalf = 0.49999999999999994;  // 0.5 - 2^-54
     double p = __builtin_fma(t1, static_cast<double>(i), t0);
-    double h = kBelowHalf;
-    if constexpr (std::is_signed<T>::value) h = __builtin_copysign(kBelowHalf, p);
-    else p = __builtin_fmax(0.0, p);
-    const double r = __builtin_trunc(p + h);
+    double r;
+    if constexpr (std::is_signed<T>::value) r = round_away(p);
+    else r = __builtin_trunc(__builtin_fmax(0.0, p) + 0.49999999999999994);  // p >= 0 after the clamp
     if constexpr (sizeof(T) == 8) {
         return static_cast<T>(__double_as_longlong(r + 6755399441055744.0) - 0x4338000000000000LL);
     } else {
@@ -175,7 +181,7 @@ __device__ __forceinline__ T dec_pred_poly(int32_t type, double t0, double t1, d
         p = __builtin_fmax(0.0, p);
         if constexpr (sizeof(T) == 4) p = __builtin_fmin(p, 4294967295.0);
     }
-    return sat_cast<T>(__builtin_round(p));
+    return sat_cast<T>(round_away(p));
 }
 
 // Extract `bits` (0..64) bits at bit position `rel` of a word window `w`

@@ -124,7 +124,7 @@ __device__ void sm_theta(i128 Sy, i128 Sxy, int64_t n, double& t0, double& t1) {
 
 template <class T> __device__ __forceinline__ int64_t abs_err(T v, double t0, double t1, int64_t i) {
     const double pr = t0 + t1 * static_cast<double>(i);  // no contraction (-ffp-contract=off)
-    const int64_t d = calc_delta<T>(v, x86_cast<T>(__builtin_round(pr)));  // the host cost model's conversion
+    const int64_t d = calc_delta<T>(v, x86_cast<T>(round_away(pr)));  // the host cost model's conversion
     return d < 0 ? static_cast<int64_t>(0ULL - static_cast<uint64_t>(d)) : d;
 }
 
@@ -225,7 +225,7 @@ __device__ double wg_cost(const T* __restrict__ v, int64_t s, int64_t e, FitLds&
         const T x = v[s + i];
         if (fast && small_value<T>(x)) {
             const double pr = t0 + t1 * static_cast<double>(i);  // no contraction (-ffp-contract=off)
-            md = __builtin_fmax(md, __builtin_fabs(static_cast<double>(x) - __builtin_round(pr)));
+            md = __builtin_fmax(md, __builtin_fabs(static_cast<double>(x) - round_away(pr)));
         } else {
             const int64_t a = abs_err<T>(x, t0, t1, i);
             mx = a > mx ? a : mx;
@@ -325,7 +325,7 @@ __global__ __launch_bounds__(kSmThreads) void k_sm_split(const T* __restrict__ v
             for (int k = 0; k < kMaxCand; ++k) {
                 if (k < K && i < static_cast<int64_t>(kCandUnits[k]) * m) {
                     const double pr = s_t[k][0] + s_t[k][1] * xi;  // no contraction (-ffp-contract=off)
-                    md[k] = __builtin_fmax(md[k], __builtin_fabs(y - __builtin_round(pr)));
+                    md[k] = __builtin_fmax(md[k], __builtin_fabs(y - round_away(pr)));
                 }
             }
         }
@@ -561,7 +561,7 @@ __global__ __launch_bounds__(kSmThreads) void k_sm_split_lds(const T* __restrict
                 for (int i = tid; i < size; i += kSmThreads) {
                     const double pr = t0 + t1 * static_cast<double>(i);  // no contraction (-ffp-contract=off)
                     md = __builtin_fmax(md, __builtin_fabs(__longlong_as_double(static_cast<long long>(sv[i])) -
-                                                           __builtin_round(pr)));
+                                                           round_away(pr)));
                 }
                 mx = static_cast<int64_t>(md);
             } else {

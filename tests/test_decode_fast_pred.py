@@ -57,6 +57,17 @@ def test_rounding_identity():
     p = adversarial_doubles(np.random.default_rng(0))
     fast = np.trunc(p + np.copysign(C, p))
     assert np.array_equal(fast, round_half_away(p))
+    # round_away replaces the library round everywhere (the saturating
+    # conversions, split/merge's cost passes), so it must hold beyond 2^53 and
+    # for the special values too
+    rng = np.random.default_rng(2)
+    big = np.concatenate([np.ldexp(rng.uniform(1, 2, 20000), rng.integers(50, 1024, 20000)),
+                          2.0 ** np.arange(50, 1024), np.nextafter(2.0 ** np.arange(50, 1024), 0)])
+    big = np.concatenate([big, -big, [np.inf, -np.inf, np.finfo(np.float64).max, -np.finfo(np.float64).max]])
+    with np.errstate(invalid="ignore"):
+        assert np.array_equal(np.trunc(big + np.copysign(C, big)), np.where(np.isinf(big), big, round_half_away(big)))
+        nan = np.array([np.nan])
+        assert np.isnan(np.trunc(nan + np.copysign(C, nan)))[0]
     # the unsigned form: no copysign after fmax(0, p)
     q = np.fmax(0.0, p)
     assert np.array_equal(np.trunc(q + C), round_half_away(q))

@@ -461,7 +461,10 @@ def mixed_kind_column(dt, n, S, seed):
 
 
 @pytest.mark.parametrize("dt", [np.int32, np.uint64, np.int64])
-@pytest.mark.parametrize("S", [1024, 512, 2048, 1000, 3000])
+# the fit's forms by partition size: <= 1024 and <= 2048 values one wave from
+# registers, 2049-4096 and 4097-8192 one workgroup from registers (fixed
+# 2048-4096 through k_fit_reg), above 8192 the two-pass wave form
+@pytest.mark.parametrize("S", [1024, 512, 2048, 1000, 3000, 4096, 6000, 8192, 9000])
 def test_parity_mixed_partition_kinds(gpu, dt, S):
     n = 40 * 1024 + 37
     print(f"mixed kinds dtype {np.dtype(dt).name} S {S}", flush=True)
@@ -493,8 +496,10 @@ def test_v4_blob_reads_like_v5(gpu, gen):
     assert np.array_equal(h.random_access_cpu(pos), want)
 
 
-@pytest.mark.parametrize("gen,part", [("u64", "v6"), ("i64", "fixed"), ("i32", "fixed"), ("u32", "v6")])
-def test_parity_fit_form_1(gpu, gen, part):
+@pytest.mark.parametrize("gen,part,size", [("u64", "v6", 0), ("i64", "fixed", 4096), ("i32", "fixed", 4096),
+                                           ("u32", "v6", 0), ("i64", "fixed", 1000), ("u64", "fixed", 6000),
+                                           ("i64", "fixed", 12000)])
+def test_parity_fit_form_1(gpu, gen, part, size):
     """glc_opts.fit_form = 1 (no fma in the fit's sums, determinant, numerator
     and intercept; compression/CompressionKernels.cuh:113-158 as written
     without contraction) against the oracle's form 1, bit for bit, on values
@@ -503,15 +508,14 @@ def test_parity_fit_form_1(gpu, gen, part):
     n = 200_000
     v = GENS[gen](n)
     gp, op = PART[part]
-    size = 4096 if part == "fixed" else 0
     col = glc.compress(to_dev(v), gp, size, 0, fit_form=1)
-    ref = O.compress(v, op, 4096, 2048, fit_form=1)
+    ref = O.compress(v, op, size or 4096, 2048, fit_form=1)
     assert_same_column(col, ref)
     assert col.serialize() == ref.blob
     assert np.array_equal(host(col.decompress_full()), v)
     pos = np.random.default_rng(3).integers(0, n, 4000).astype(np.int32)
     assert np.array_equal(host(col.random_access(to_dev(pos))), v[pos])
-    ref0 = O.compress(v, op, 4096, 2048, fit_form=0)
+    ref0 = O.compress(v, op, size or 4096, 2048, fit_form=0)
     if gen in ("u64", "i64"):  # sums above 2^53: the contraction shows in theta
         assert not np.array_equal(ref0.params.view(np.uint64), ref.params.view(np.uint64))
     with pytest.raises(glc.GlcError):
