@@ -156,13 +156,14 @@ __device__ __forceinline__ double cost_of(int64_t n, int32_t bits) {
     return 80.0 + static_cast<double>(n) * bits / 8.0;  // CpuPartitioner.cuh:128-138
 }
 
-struct FitLds {
-    i128 sy[kSmThreads / 64];
-    i128 sxy[kSmThreads / 64];
-    int64_t mx[kSmThreads / 64];
+template <int NT> struct FitLdsT {
+    i128 sy[NT / 64];
+    i128 sxy[NT / 64];
+    int64_t mx[NT / 64];
     double t0, t1, cost;
     int fast;  // both end predictions within small_pred's range
 };
+typedef FitLdsT<kSmThreads> FitLds;
 
 // Block prefix sums of the column (k_sm_block_pfx) when they exist (p1 null
 // otherwise): G1(p), G2(p) over the first min(p m, n) values.
@@ -179,8 +180,9 @@ __device__ __forceinline__ i128 pfx_at(const i128* p, const i128* c, int64_t i) 
 // start or the column's end (every split/merge boundary is), otherwise from a
 // pass over the values.  The maximum residual takes the exact FP64 form for
 // values within small_value's range when both end predictions are in range.
-template <class T>
-__device__ double wg_cost(const T* __restrict__ v, int64_t s, int64_t e, FitLds& L, const SmPrefix& pf) {
+template <class T, int NT = kSmThreads>
+__device__ double wg_cost(const T* __restrict__ v, int64_t s, int64_t e, FitLdsT<NT>& L,
+                          const SmPrefix& pf) {
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int64_t n = e - s;
     const bool use_pfx = pf.p1 != nullptr && s % pf.m == 0 && (e == pf.n || e % pf.m == 0);
@@ -194,7 +196,7 @@ __device__ double wg_cost(const T* __restrict__ v, int64_t s, int64_t e, FitLds&
         }
     } else {
         i128 sy = 0, sxy = 0;
-        for (int64_t i = tid; i < n; i += kSmThreads) {
+        for (int64_t i = tid; i < n; i += NT) {
             const i128 y = wide<T>(v[s + i]);
             sy += y;
             sxy += y * static_cast<i128>(i);
@@ -208,7 +210,8 @@ __device__ double wg_cost(const T* __restrict__ v, int64_t s, int64_t e, FitLds&
         __syncthreads();
         if (tid == 0) {
             i128 a = 0, b = 0;
-            for (int k = 0; k < kSmThreads / 64; ++k) {
+#pragma unroll 1
+            for (int k = 0; k < NT / 64; ++k) {
                 a += L.sy[k];
                 b += L.sxy[k];
             }
@@ -221,14 +224,24 @@ __device__ double wg_cost(const T* __restrict__ v, int64_t s, int64_t e, FitLds&
     const bool fast = L.fast != 0;
     int64_t mx = 0;
     double md = 0.0;
-    for (int64_t i = tid; i < n; i += kSmThreads) {
-        const T x = v[s + i];
-        if (fast && small_value<T>(x)) {
-            const double pr = t0 + t1 * static_cast<double>(i);  // no contraction (-ffp-contract=off)
-            md = __builtin_fmax(md, __builtin_fabs(static_cast<double>(x) - round_away(pr)));
-        } else {
-            const int64_t a = abs_err<T>(x, t0, t1, i);
-            mx = a > mx ? a : mx;
+    for (int64_t i0 = tid; i0 < n; i0 += 4 * NT) {
+        T x[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {  // four loads in flight (index clamped; the extra ones are skipped)
+            const int64_t i = i0 + k * NT;
+            x[k] = v[s + (i < n ? i : n - 1)];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int64_t i = i0 + k * NT;
+            if (i >= n) break;
+            if (fast && small_value<T>(x[k])) {
+                const double pr = t0 + t1 * static_cast<double>(i);  // no contraction (-ffp-contract=off)
+                md = __builtin_fmax(md, __builtin_fabs(static_cast<double>(x[k]) - round_away(pr)));
+            } else {
+                const int64_t a = abs_err<T>(x[k], t0, t1, i);
+                mx = a > mx ? a : mx;
+            }
         }
     }
     const int64_t mdi = static_cast<int64_t>(md);
@@ -238,7 +251,8 @@ __device__ double wg_cost(const T* __restrict__ v, int64_t s, int64_t e, FitLds&
     __syncthreads();
     if (tid == 0) {
         int64_t m = L.mx[0];
-        for (int k = 1; k < kSmThreads / 64; ++k) m = L.mx[k] > m ? L.mx[k] : m;
+#pragma unroll 1
+        for (int k = 1; k < NT / 64; ++k) m = L.mx[k] > m ? L.mx[k] : m;
         L.cost = cost_of(n, bits_of(m));
     }
     __syncthreads();
@@ -826,10 +840,10 @@ __global__ __launch_bounds__(kSmThreads) void k_sm_seg(const T* __restrict__ v, 
 
 // The group that starts at partition e: returns the index of the first
 // partition after it and its cost (all threads call it).
-template <class T>
+template <class T, int NT>
 __device__ int64_t sm_grow(const T* __restrict__ v, const int32_t* __restrict__ st, const int32_t* __restrict__ en,
                            const double* __restrict__ cost, const double* __restrict__ mc,
-                           const int32_t* __restrict__ flag, int64_t P, int64_t e, double& gc_out, FitLds& L,
+                           const int32_t* __restrict__ flag, int64_t P, int64_t e, double& gc_out, FitLdsT<NT>& L,
                            const SmPrefix& pf) {
     const int64_t gs = st[e];
     double gc = cost[e];
@@ -843,7 +857,7 @@ __device__ int64_t sm_grow(const T* __restrict__ v, const int32_t* __restrict__ 
             merge = flag[j - 1] != 0;
             merged = mc[j - 1];
         } else {
-            merged = wg_cost<T>(v, gs, en[j], L, pf);
+            merged = wg_cost<T, NT>(v, gs, en[j], L, pf);
             merge = merged < gc + cj;
         }
         if (!merge) break;
@@ -855,8 +869,12 @@ __device__ int64_t sm_grow(const T* __restrict__ v, const int32_t* __restrict__ 
 }
 
 constexpr int kFixWin = 2048;
+// One workgroup of 16 waves (4 per SIMD): its regrowth scans are the only
+// work on the GPU while it runs, and 4 waves per SIMD hide their load latency
+// where 1 did not.
+constexpr int kFixThreads = 1024;
 template <class T>
-__global__ __launch_bounds__(kSmThreads) void k_sm_fix(const T* __restrict__ v, const int32_t* __restrict__ st,
+__global__ __launch_bounds__(kFixThreads) void k_sm_fix(const T* __restrict__ v, const int32_t* __restrict__ st,
                                                        const int32_t* __restrict__ en,
                                                        const double* __restrict__ cost,
                                                        const double* __restrict__ mc,
@@ -864,7 +882,7 @@ __global__ __launch_bounds__(kSmThreads) void k_sm_fix(const T* __restrict__ v, 
                                                        int64_t K, int32_t* __restrict__ spec,
                                                        int32_t* __restrict__ gend, double* __restrict__ gcost,
                                                        const int32_t* __restrict__ exitk, SmPrefix pf) {
-    __shared__ FitLds L;
+    __shared__ FitLdsT<kFixThreads> L;
     __shared__ int32_t s_spec[kFixWin], s_exit[kFixWin];
     const int tid = threadIdx.x;
     // Every write below is to an entry before the walk's position e, every
@@ -876,33 +894,33 @@ __global__ __launch_bounds__(kSmThreads) void k_sm_fix(const T* __restrict__ v, 
         const int64_t a0 = k * seg;
         const int64_t a1 = (a0 + seg < P) ? a0 + seg : P;
         const int64_t c = e < a1 ? e : a1;
-        for (int64_t a = a0 + tid; a < c; a += kSmThreads) spec[a] = 0;  // inside the group before e
+        for (int64_t a = a0 + tid; a < c; a += kFixThreads) spec[a] = 0;  // inside the group before e
         while (e < a1) {
             if (wb < 0 || e >= wb + kFixWin) {
                 __syncthreads();
                 wb = e;
-                for (int q = tid; q < kFixWin; q += kSmThreads) s_spec[q] = (wb + q < P) ? spec[wb + q] : 0;
+                for (int q = tid; q < kFixWin; q += kFixThreads) s_spec[q] = (wb + q < P) ? spec[wb + q] : 0;
                 __syncthreads();
             }
             if (s_spec[e - wb]) {  // the segment's own walk from e is the true one
                 if (xb < 0 || k >= xb + kFixWin) {
                     __syncthreads();
                     xb = k;
-                    for (int q = tid; q < kFixWin; q += kSmThreads) s_exit[q] = (xb + q < K) ? exitk[xb + q] : 0;
+                    for (int q = tid; q < kFixWin; q += kFixThreads) s_exit[q] = (xb + q < K) ? exitk[xb + q] : 0;
                     __syncthreads();
                 }
                 e = s_exit[k - xb];
                 break;
             }
             double gc;
-            const int64_t j = sm_grow<T>(v, st, en, cost, mc, flag, P, e, gc, L, pf);
+            const int64_t j = sm_grow<T, kFixThreads>(v, st, en, cost, mc, flag, P, e, gc, L, pf);
             if (tid == 0) {
                 spec[e] = 1;
                 gend[e] = static_cast<int32_t>(j);
                 gcost[e] = gc;
             }
             const int64_t hi = j < a1 ? j : a1;
-            for (int64_t a = e + 1 + tid; a < hi; a += kSmThreads) spec[a] = 0;
+            for (int64_t a = e + 1 + tid; a < hi; a += kFixThreads) spec[a] = 0;
             e = j;
         }
     }
@@ -1065,7 +1083,7 @@ int gpu_split_merge(int dtype, const void* d_values, int64_t n, int32_t m, std::
             const int64_t K = (P + sg - 1) / sg;
             hipLaunchKernelGGL(k_sm_seg<T>, dim3(static_cast<unsigned>(K)), dim3(kSmThreads), 0, s, v, sa, ea, ca, mc,
                                flag, P, sg, spec, gend, gcost, exitk, pf);
-            hipLaunchKernelGGL(k_sm_fix<T>, dim3(1), dim3(kSmThreads), 0, s, v, sa, ea, ca, mc, flag, P, sg, K, spec,
+            hipLaunchKernelGGL(k_sm_fix<T>, dim3(1), dim3(kFixThreads), 0, s, v, sa, ea, ca, mc, flag, P, sg, K, spec,
                                gend, gcost, exitk, pf);
             scan_flags(spec, P);
             hipLaunchKernelGGL(k_sm_emit_groups, dim3(blocks(P, 256)), dim3(256), 0, s, spec, pos, gend, gcost, sa, ea,
