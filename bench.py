@@ -313,7 +313,7 @@ def bench_i32(args):
         "vs_baseline": None, "dtype": "int32", "data": "synthetic (seeded, generated on the GPU)",
         "config": {"workload": f"int32_linear_noise13_N2^{args.log2n}_fixed4096", "n": n,
                    "partitions": inf.num_partitions, "compressed_bytes": inf.compressed_size,
-                   "decode_schedule": SCHEDULE_NAMES[sched]},
+                   "decode_schedule": SCHEDULE_NAMES[sched], "decode_pace": col.decode_pace()},
         "roofline": {"bound": "hbm", "achieved": round(alg / (ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                      "traffic": decode_traffic("i32", sched)[0], "traffic_source": decode_traffic("i32", sched)[1],
@@ -384,7 +384,7 @@ def bench_c3(args):
         "data": "synthetic (seeded, generated on the GPU)",
         "config": {"workload": f"c3_int64_sorted_timestamps_N2^{args.log2n}_fixed4096", "n": n, "partitions": P,
                    "compressed_bytes": inf.compressed_size, "lookups": q, "lookup_mix": "uniform",
-                   "decode_schedule": SCHEDULE_NAMES[sched]},
+                   "decode_schedule": SCHEDULE_NAMES[sched], "decode_pace": col.decode_pace()},
         "ms": {k: round(v, 4) for k, v in res.items()},
         "full_GBps": {"packed": round(8 * n / res["packed_full_ms"] / 1e6, 1),
                       "packed_fix": round(8 * n / res["packed_fix_full_ms"] / 1e6, 1),
@@ -846,7 +846,7 @@ def main():
     # the column's decode schedule, measured on this column (setup, like the
     # batched form's autotuner)
     sched, sched_ms = col.autotune_decode(out, reps=10)
-    log(f"decode schedule {SCHEDULE_NAMES[sched]} ({sched_ms:.4f} ms median in the tuner)")
+    log(f"decode schedule {SCHEDULE_NAMES[sched]}, pace {col.decode_pace()} ({sched_ms:.4f} ms median in the tuner)")
     for _ in range(args.warmup):
         col.decompress_full(mode, out=out)
         col.random_access(pos, out=look)
@@ -943,7 +943,7 @@ def main():
         "data": "synthetic (seeded generator of the configured shape, generated on the GPU)",
         "config": {"workload": f"c2_uint64_piecewise_linear_N2^{args.log2n}_v6", "n": n, "partitioner": "v6",
                    "v6_base": 2048, "decoder": DECODER_KERNEL[args.mode].format(K=SCHEDULE_KERNEL[sched].format(T="uint64")),
-                   "decode_schedule": SCHEDULE_NAMES[sched],
+                   "decode_schedule": SCHEDULE_NAMES[sched], "decode_pace": col.decode_pace(),
                    "lookups_per_step": q,
                    "lookup_mix": "1/2 uniform + 1/2 zipf(1.1) scrambled", "partitions": inf.num_partitions,
                    "compressed_bytes": inf.compressed_size, "compression_ratio": round(n * 8 / inf.compressed_size, 4)},

@@ -701,6 +701,20 @@ int glc_column_get_decode_schedule(const glc_column* c, int32_t* schedule) {
     return GLC_OK;
 }
 
+int glc_column_set_decode_pace(glc_column* c, int32_t pace) {
+    if (!c) return fail(GLC_ERR_INVALID_ARG, "NULL column");
+    if (pace != 0 && pace != 8 && pace != 16 && pace != 24)
+        return fail(GLC_ERR_INVALID_ARG, "decode pace must be 0, 8, 16 or 24");
+    c->dec_pace = pace;
+    return GLC_OK;
+}
+
+int glc_column_get_decode_pace(const glc_column* c, int32_t* pace) {
+    if (!c || !pace) return fail(GLC_ERR_INVALID_ARG, "NULL argument");
+    *pace = c->dec_pace;
+    return GLC_OK;
+}
+
 int glc_column_autotune_decode(glc_column* c, void* d_out, int32_t reps, void* stream, int32_t* schedule, float* ms) {
     if (!c) return fail(GLC_ERR_INVALID_ARG, "NULL column");
     if (c->n > 0 && !d_out) return fail(GLC_ERR_INVALID_ARG, "NULL output");

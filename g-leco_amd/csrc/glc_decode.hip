@@ -48,6 +48,7 @@ struct DecArgs {
     int64_t wlimit;                      // payload words (row-load base clamp)
     int32_t* sched;                      // per-XCD work-stealing queues (wave schedule)
     int32_t vec16;                       // out is 16-byte aligned: narrow tiles store 16 bytes per lane
+    int32_t pace;                        // pause after each narrow row's stores (pace_row)
 };
 
 // Decode elements [e0, e1) of partition `d` (caller guarantees e0 >= d.start
@@ -244,9 +245,23 @@ __device__ __forceinline__ void wave_narrow(const DecArgs& a, const TileInfo& x,
 // instead of 16 for 32-bit values with fields of at most 16 bits.
 // NP = 1: the whole tile; NP = 2: half h of the tile's rows (the half-tile
 // schedule, one wave per half): rows rbase ...
-// FAST: every prediction of the wave's rows lies in dec_fast_range; FULL:
-// every row lies inside the column (no per-row end check).
-template <class T, int OP, int V, bool FAST, bool FULL, int NR>
+// Pause after a row's stores: the column's decode pace, in units of 64
+// cycles (0 = none).  Less arithmetic between a wave's stores made some
+// decodes slower; a pause restores the spacing.  On the 4 GiB int64
+// timestamp column the half-tile decode ran 0.945 ms back to back with no
+// pause and 0.88-0.90 ms with 16 per row (32: 1.00 ms); the uint64 2^28
+// column and the int32 column are fastest without.  The autotuner picks it
+// with the schedule.
+__device__ __forceinline__ void pace_row(int32_t pace) {
+    if (pace == 8) __builtin_amdgcn_s_sleep(8);
+    else if (pace == 16) __builtin_amdgcn_s_sleep(16);
+    else if (pace == 24) __builtin_amdgcn_s_sleep(24);
+}
+
+// FAST: 0 = the saturating conversion (dec_pred); 1 or 2 = every prediction
+// of the wave's rows lies in dec_fast_range of that tier; FULL: every row
+// lies inside the column (no per-row end check).
+template <class T, int OP, int V, int FAST, bool FULL, int NR>
 __device__ __forceinline__ void narrow_v_rows(const DecArgs& a, const TileInfo& x, int64_t e0, int64_t e1, int lane,
                                               int rbase, const uint32_t (&x0)[NR], const uint32_t (&x1)[NR],
                                               const uint32_t (&xs)[NR]) {
@@ -283,7 +298,7 @@ __device__ __forceinline__ void narrow_v_rows(const DecArgs& a, const TileInfo& 
                                                   static_cast<int>(raw), 0u, static_cast<unsigned>(b))));
             const int32_t i = lbase + r * kRowVals + lane * V + k;
             if constexpr (OP == OP_UNPACK) v[k] = delta;
-            else if constexpr (FAST) v[k] = apply_delta<T>(dec_pred_fast<T>(x.t0, x.t1, i), delta);
+            else if constexpr (FAST != 0) v[k] = apply_delta<T>(dec_pred_fast<T, FAST>(x.t0, x.t1, i), delta);
             else v[k] = apply_delta<T>(dec_pred<T>(x.t0, x.t1, i), delta);
         }
         const int64_t e = e0 + r * kRowVals + lane * V;
@@ -315,17 +330,21 @@ __device__ __forceinline__ void narrow_v_rows(const DecArgs& a, const TileInfo& 
             for (int k = 0; k < V; ++k)
                 if (e + k < e1) put(out + e + k, v[k]);
         }
+        if constexpr (FAST != 0) pace_row(a.pace);
     }
 }
 
 // True when every prediction of local indices i0 .. i1 of the tile's
-// partition lies in dec_fast_range (the same on every lane).
-template <class T>
+// partition lies in dec_fast_range<T, TIER> (the same on every lane).
+template <class T, int TIER = 1>
 __device__ __forceinline__ bool tile_fast(const TileInfo& x, int32_t i0, int32_t i1) {
-    const bool f = dec_fast_range<T>(__builtin_fma(x.t1, static_cast<double>(i0), x.t0),
-                                     __builtin_fma(x.t1, static_cast<double>(i1), x.t0));
+    const bool f = dec_fast_range<T, TIER>(__builtin_fma(x.t1, static_cast<double>(i0), x.t0),
+                                           __builtin_fma(x.t1, static_cast<double>(i1), x.t0));
     return __builtin_amdgcn_readfirstlane(static_cast<int>(f)) != 0;
 }
+
+template <class T, int OP>
+__device__ __forceinline__ void wave_slow(const DecArgs& a, int64_t e0, int64_t e1, int64_t plo, int lane);
 
 template <class T, int OP, int V, int NP = 1>
 __device__ __forceinline__ void wave_narrow_v(const DecArgs& a, const TileInfo& x, int64_t e0, int64_t e1, int lane,
@@ -363,15 +382,30 @@ __device__ __forceinline__ void wave_narrow_v(const DecArgs& a, const TileInfo& 
     }
     if constexpr (OP == OP_PACKED) {
         const int32_t i0 = x.lbase + rbase * kRowVals;
-        if (tile_fast<T>(x, i0, i0 + kRows * kRowVals - 1)) {
-            if (e0 + (rbase + kRows) * kRowVals <= e1)
-                narrow_v_rows<T, OP, V, true, true>(a, x, e0, e1, lane, rbase, x0, x1, xs);
-            else
-                narrow_v_rows<T, OP, V, true, false>(a, x, e0, e1, lane, rbase, x0, x1, xs);
+        const int32_t i1 = i0 + kRows * kRowVals - 1;
+        const bool full = e0 + (rbase + kRows) * kRowVals <= e1;
+        if (tile_fast<T, 1>(x, i0, i1)) {
+            if (full) narrow_v_rows<T, OP, V, 1, true>(a, x, e0, e1, lane, rbase, x0, x1, xs);
+            else narrow_v_rows<T, OP, V, 1, false>(a, x, e0, e1, lane, rbase, x0, x1, xs);
             return;
         }
+        if constexpr (sizeof(T) == 8) {  // 64-bit predictions beyond 2^50 (timestamps, large offsets)
+            if (tile_fast<T, 2>(x, i0, i1)) {
+                if (full) narrow_v_rows<T, OP, V, 2, true>(a, x, e0, e1, lane, rbase, x0, x1, xs);
+                else narrow_v_rows<T, OP, V, 2, false>(a, x, e0, e1, lane, rbase, x0, x1, xs);
+                return;
+            }
+        }
+        // predictions that saturate (beyond T, NaN): the general per-lane path
+        // over this wave's rows.  Carrying the saturating conversion through
+        // the unrolled rows instead cost ~8 KB of code per element type for a
+        // case only extreme fits reach.
+        const int64_t p0 = e0 + rbase * kRowVals;
+        const int64_t p1 = (e0 + (rbase + kRows) * kRowVals < e1) ? e0 + (rbase + kRows) * kRowVals : e1;
+        wave_slow<T, OP>(a, p0, p1, a.tile_part[e0 >> kIndexShift], lane);
+    } else {
+        narrow_v_rows<T, OP, V, 0, false>(a, x, e0, e1, lane, rbase, x0, x1, xs);
     }
-    narrow_v_rows<T, OP, V, false, false>(a, x, e0, e1, lane, rbase, x0, x1, xs);
 }
 
 // Wide fast path: one partition, 33..64-bit fields or DIRECT_COPY; two row
@@ -443,7 +477,7 @@ __device__ __forceinline__ void wave_plain(const DecArgs& a, const TileInfo& x, 
 // of 128 values.  The delta array is a device allocation (16-byte aligned)
 // and e is even, so the vector loads are aligned; the output's alignment is
 // the caller's (VEC instantiation).  PRED: 0 = DIRECT_COPY, 1 = dec_pred,
-// 2 = dec_pred_fast.
+// 2 / 3 = dec_pred_fast of tier 1 / 2.
 typedef int64_t i64x2 __attribute__((ext_vector_type(2)));
 constexpr int kPlainRows = kWaveTile / 128;
 
@@ -464,8 +498,8 @@ __device__ __forceinline__ void plain_v_rows(const DecArgs& a, const TileInfo& x
             v[0] = apply_delta<T>(dec_pred<T>(x.t0, x.t1, i), pl[r].x);
             v[1] = apply_delta<T>(dec_pred<T>(x.t0, x.t1, i + 1), pl[r].y);
         } else {
-            v[0] = apply_delta<T>(dec_pred_fast<T>(x.t0, x.t1, i), pl[r].x);
-            v[1] = apply_delta<T>(dec_pred_fast<T>(x.t0, x.t1, i + 1), pl[r].y);
+            v[0] = apply_delta<T>(dec_pred_fast<T, PRED - 1>(x.t0, x.t1, i), pl[r].x);
+            v[1] = apply_delta<T>(dec_pred_fast<T, PRED - 1>(x.t0, x.t1, i + 1), pl[r].y);
         }
         if (e + 2 <= e1) {
             if constexpr (sizeof(T) == 8) {
@@ -503,7 +537,9 @@ __device__ __forceinline__ void wave_plain_v(const DecArgs& a, const TileInfo& x
         }
     }
     if (tile_type(x.info) == MODEL_DIRECT_COPY) plain_v_rows<T, 0>(a, x, e0, e1, lane, pl);
-    else if (tile_fast<T>(x, x.lbase, x.lbase + kWaveTile - 1)) plain_v_rows<T, 2>(a, x, e0, e1, lane, pl);
+    else if (tile_fast<T, 1>(x, x.lbase, x.lbase + kWaveTile - 1)) plain_v_rows<T, 2>(a, x, e0, e1, lane, pl);
+    else if (sizeof(T) == 8 && tile_fast<T, 2>(x, x.lbase, x.lbase + kWaveTile - 1))
+        plain_v_rows<T, 3>(a, x, e0, e1, lane, pl);
     else plain_v_rows<T, 1>(a, x, e0, e1, lane, pl);
 }
 
@@ -656,6 +692,18 @@ __global__ void k_zero_queues(int32_t* q) {
 // Batched decode: wave (w, k) of workgroup g decodes global tile
 // (g*WPG + w)*TPW + k of the batch; the column comes from the tile map and the
 // column table (scalar loads), and the element type selects the code path.
+// A batch column whose output is not 16-byte aligned: the general per-lane
+// path (wave_slow handles every tile kind).  Keeping the one-value-per-lane
+// tile bodies out of the batch kernels halves their code (152 KB with them
+// for the four element types, more than the instruction cache), and such
+// outputs are the exception.
+template <class T>
+__device__ __forceinline__ void batch_tile_unaligned(const DecArgs& a, int64_t t, int lane) {
+    const int64_t e0 = t * kWaveTile;
+    const int64_t e1 = (e0 + kWaveTile < a.n) ? e0 + kWaveTile : a.n;
+    wave_slow<T, OP_PACKED>(a, e0, e1, a.tile_part[t], lane);
+}
+
 template <int WPG, int TPW>
 __global__ __launch_bounds__(64 * WPG) void k_batch_decode(const BatchCol* __restrict__ cols,
                                                            const int32_t* __restrict__ tile_col,
@@ -682,6 +730,7 @@ __global__ __launch_bounds__(64 * WPG) void k_batch_decode(const BatchCol* __res
         a.sched = nullptr;
         a.wlimit = c.payload_words;
         a.vec16 = (reinterpret_cast<uintptr_t>(a.out) & 15) == 0;
+        a.pace = 0;
         const int64_t t = g - c.tile_base;
         if (a.vec16) {
             switch (c.dtype) {
@@ -692,10 +741,10 @@ __global__ __launch_bounds__(64 * WPG) void k_batch_decode(const BatchCol* __res
             }
         } else {
             switch (c.dtype) {
-                case GLC_I32: wave_tile<int32_t, OP_PACKED, false>(a, t, lane); break;
-                case GLC_U32: wave_tile<uint32_t, OP_PACKED, false>(a, t, lane); break;
-                case GLC_I64: wave_tile<int64_t, OP_PACKED, false>(a, t, lane); break;
-                default: wave_tile<uint64_t, OP_PACKED, false>(a, t, lane); break;
+                case GLC_I32: batch_tile_unaligned<int32_t>(a, t, lane); break;
+                case GLC_U32: batch_tile_unaligned<uint32_t>(a, t, lane); break;
+                case GLC_I64: batch_tile_unaligned<int64_t>(a, t, lane); break;
+                default: batch_tile_unaligned<uint64_t>(a, t, lane); break;
             }
         }
     }
@@ -710,7 +759,7 @@ __device__ __forceinline__ void batch_half(const DecArgs& a, int64_t t, int h, i
     if (tile_path(x.info) != TILE_NARROW || !a.vec16) {
         if (h == 0) {
             if (a.vec16) wave_tile<T, OP_PACKED, true>(a, t, lane);
-            else wave_tile<T, OP_PACKED, false>(a, t, lane);
+            else batch_tile_unaligned<T>(a, t, lane);
         }
         return;
     }
@@ -745,6 +794,7 @@ __global__ __launch_bounds__(64) void k_batch_decode_half(const BatchCol* __rest
     a.sched = nullptr;
     a.wlimit = c.payload_words;
     a.vec16 = (reinterpret_cast<uintptr_t>(a.out) & 15) == 0;
+    a.pace = 0;
     const int64_t t = g - c.tile_base;
     switch (c.dtype) {
         case GLC_I32: batch_half<int32_t>(a, t, h, lane); break;
@@ -792,6 +842,7 @@ DecArgs make_args(const glc_column* c, void* out) {
     a.wlimit = c->payload_words;
     a.sched = c->d_sched;
     a.vec16 = (reinterpret_cast<uintptr_t>(out) & 15) == 0;
+    a.pace = c->dec_pace;
     return a;
 }
 
@@ -887,10 +938,12 @@ int glc_autotune_decode(glc_column* c, void* d_out, int32_t reps, hipStream_t s,
         if (ms_out) *ms_out = 0.0f;
         return GLC_OK;
     }
-    constexpr int kS = 3;
+    // every schedule with every pace
+    constexpr int kS = 3, kPaces = 4, kC = kS * kPaces;
     const int sched_of[kS] = {GLC_DECODE_TILE_W1, GLC_DECODE_TILE_W4, GLC_DECODE_HALF_TILE_W1};
-    const int saved = c->dec_sched;
-    std::vector<hipEvent_t> ev(static_cast<size_t>(2 * kS * reps), nullptr);
+    const int pace_of[kPaces] = {0, 8, 16, 24};
+    const int saved = c->dec_sched, saved_pace = c->dec_pace;
+    std::vector<hipEvent_t> ev(static_cast<size_t>(2 * kC * reps), nullptr);
     int rc = GLC_OK;
     for (auto& e : ev)
         if (hipEventCreate(&e) != hipSuccess) {
@@ -898,43 +951,52 @@ int glc_autotune_decode(glc_column* c, void* d_out, int32_t reps, hipStream_t s,
             rc = fail(GLC_ERR_HIP, "hipEventCreate failed");
             break;
         }
-    // one warm-up launch per schedule, then `reps` rounds over the schedules
-    // (round-robin, so clock and cache drift falls on every schedule alike)
-    for (int k = 0; k < kS && !rc; ++k) {
-        c->dec_sched = sched_of[k];
+    auto set = [&](int k) {
+        c->dec_sched = sched_of[k / kPaces];
+        c->dec_pace = pace_of[k % kPaces];
+    };
+    // one warm-up launch per combination, then `reps` rounds over them
+    // (round-robin, so clock and cache drift falls on every one alike)
+    for (int k = 0; k < kC && !rc; ++k) {
+        set(k);
         rc = launch_decode_full(c, d_out, GLC_FULL_BITPACKED, s);
     }
     for (int r = 0; r < reps && !rc; ++r)
-        for (int k = 0; k < kS && !rc; ++k) {
-            c->dec_sched = sched_of[k];
-            const size_t i = static_cast<size_t>(2 * (r * kS + k));
+        for (int k = 0; k < kC && !rc; ++k) {
+            set(k);
+            const size_t i = static_cast<size_t>(2 * (r * kC + k));
             if (hipEventRecord(ev[i], s) != hipSuccess) rc = fail(GLC_ERR_HIP, "hipEventRecord failed");
             if (!rc) rc = launch_decode_full(c, d_out, GLC_FULL_BITPACKED, s);
             if (!rc && hipEventRecord(ev[i + 1], s) != hipSuccess) rc = fail(GLC_ERR_HIP, "hipEventRecord failed");
         }
     if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = fail(GLC_ERR_HIP, "hipStreamSynchronize failed");
-    int best = saved;
+    int best = -1;
     float best_ms = 0.0f;
     if (!rc) {
-        for (int k = 0; k < kS; ++k) {  // median launch time of each schedule
+        for (int k = 0; k < kC; ++k) {  // median launch time of each combination
             std::vector<float> t(static_cast<size_t>(reps));
             for (int r = 0; r < reps; ++r) {
-                const size_t i = static_cast<size_t>(2 * (r * kS + k));
+                const size_t i = static_cast<size_t>(2 * (r * kC + k));
                 (void)hipEventElapsedTime(&t[static_cast<size_t>(r)], ev[i], ev[i + 1]);
             }
             std::sort(t.begin(), t.end());
             const float med = t[t.size() / 2];
-            if (k == 0 || med < best_ms) {
+            if (best < 0 || med < best_ms) {
                 best_ms = med;
-                best = sched_of[k];
+                best = k;
             }
         }
     }
     for (auto& e : ev)
         if (e) (void)hipEventDestroy(e);
-    c->dec_sched = rc ? saved : best;
-    if (!rc && best_out) *best_out = best;
-    if (!rc && ms_out) *ms_out = best_ms;
+    if (rc || best < 0) {
+        c->dec_sched = saved;
+        c->dec_pace = saved_pace;
+        return rc;
+    }
+    set(best);
+    if (best_out) *best_out = c->dec_sched;
+    if (ms_out) *ms_out = best_ms;
     return rc;
 }
 

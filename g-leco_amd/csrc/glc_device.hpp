@@ -135,37 +135,44 @@ template <class T> __device__ __forceinline__ T dec_pred(double t0, double t1, i
 }
 
 // Decoder prediction without branches, for a run of indices whose
-// predictions lie in dec_fast_range.  The run is checked once at its two end
-// indices: fma(theta1, i, theta0) is monotonic in i, so the ends bound every
-// index between them (NaN fails the check).  Inside the range
-//   round(p) == trunc(p + copysign(0.5 - 2^-54, p))  for every double p
-//            (the sum reaches the next integer exactly when p's fraction is
-//            at least 1/2; 0.5 - 2^-54 is the largest double below 1/2),
-//   64-bit T: |round(p)| <= 2^50, converted exactly by the 1.5 * 2^52 constant,
-//   32-bit T: round(p) lies inside T and is converted by one cvt,
-//   unsigned: fmax(0, p) is the decoder's own clamp; the upper clamp of
-//            32-bit values cannot bind,
-// so dec_pred_fast<T>(t0, t1, i) == dec_pred<T>(t0, t1, i) bit for bit.
+// predictions lie in dec_fast_range<T, TIER>.  The run is checked once at its
+// two end indices: fma(theta1, i, theta0) is monotonic in i, so the ends bound
+// every index between them (NaN fails the check).  Inside the range
+//   round(p) == round_away(p) == trunc(p + copysign(0.5 - 2^-54, p)),
+//   unsigned: fmax(0, p) is the decoder's own clamp (the upper clamp of
+//            32-bit values cannot bind),
+//   TIER 1:  64-bit T: |round(p)| <= 2^50, converted exactly by the
+//            1.5 * 2^52 constant; 32-bit T: round(p) lies inside T and is
+//            converted by one cvt;
+//   TIER 2:  64-bit T only: round(p) lies inside T (int64: [-2^63, 2^63 -
+//            1024], uint64: [0, 2^64 - 2048], the largest doubles below the
+//            limits), converted exactly by the plain conversion,
+// so dec_pred_fast<T, TIER>(t0, t1, i) == dec_pred<T>(t0, t1, i) bit for bit.
 // p_lo, p_hi: fma(theta1, i, theta0) at the run's first and last index.
-template <class T> __device__ __forceinline__ bool dec_fast_range(double p_lo, double p_hi) {
+template <class T, int TIER = 1> __device__ __forceinline__ bool dec_fast_range(double p_lo, double p_hi) {
     constexpr double k50 = 1125899906842624.0;  // 2^50
     if constexpr (std::is_same<T, int32_t>::value)
         return p_lo >= -2147483647.0 && p_lo <= 2147483646.0 && p_hi >= -2147483647.0 && p_hi <= 2147483646.0;
     else if constexpr (std::is_same<T, uint32_t>::value) return p_lo <= 4294967294.0 && p_hi <= 4294967294.0;
-    else if constexpr (std::is_same<T, int64_t>::value)
-        return p_lo >= -k50 && p_lo <= k50 && p_hi >= -k50 && p_hi <= k50;
-    else return p_lo <= k50 && p_hi <= k50;
+    else if constexpr (std::is_same<T, int64_t>::value) {
+        constexpr double lo = TIER == 1 ? -k50 : -9223372036854775808.0;
+        constexpr double hi = TIER == 1 ? k50 : 9223372036854774784.0;
+        return p_lo >= lo && p_lo <= hi && p_hi >= lo && p_hi <= hi;
+    } else {
+        constexpr double hi = TIER == 1 ? k50 : 18446744073709549568.0;
+        return p_lo <= hi && p_hi <= hi;
+    }
 }
 
-template <class T> __device__ __forceinline__ T dec_pred_fast(double t0, double t1, int32_t i) {
+template <class T, int TIER = 1> __device__ __forceinline__ T dec_pred_fast(double t0, double t1, int32_t i) {
     double p = __builtin_fma(t1, static_cast<double>(i), t0);
     double r;
     if constexpr (std::is_signed<T>::value) r = round_away(p);
     else r = __builtin_trunc(__builtin_fmax(0.0, p) + 0.49999999999999994);  // p >= 0 after the clamp
-    if constexpr (sizeof(T) == 8) {
+    if constexpr (sizeof(T) == 8 && TIER == 1) {
         return static_cast<T>(__double_as_longlong(r + 6755399441055744.0) - 0x4338000000000000LL);
     } else {
-        return static_cast<T>(r);
+        return static_cast<T>(r);  // inside T: no saturation to apply
     }
 }
 

@@ -177,6 +177,7 @@ struct glc_column {
     int32_t fit_form = 0;           // contraction form of the fit (glc_opts.fit_form)
     int32_t has_poly = 0;
     int32_t dec_sched = GLC_DECODE_TILE_W4;  // packed full-decode schedule (glc_column_set_decode_schedule)
+    int32_t dec_pace = 0;                    // pause per decoded row (glc_column_set_decode_pace)
     // arena
     void* meta_arena = nullptr;
     int64_t meta_bytes = 0;

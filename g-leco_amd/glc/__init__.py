@@ -70,6 +70,8 @@ SIGNATURES = [
     ("glc_decompress_full_fixed", _int, [_vp, _vp, _i32, _vp]),
     ("glc_column_set_decode_schedule", _int, [_vp, _i32]),
     ("glc_column_get_decode_schedule", _int, [_vp, _vp]),
+    ("glc_column_set_decode_pace", _int, [_vp, _i32]),
+    ("glc_column_get_decode_pace", _int, [_vp, _vp]),
     ("glc_column_autotune_decode", _int, [_vp, _vp, _i32, _vp, _vp, _vp]),
     ("glc_random_access", _int, [_vp, _vp, _i64, _vp, _int, _vp]),
     ("glc_random_access_i64", _int, [_vp, _vp, _i64, _vp, _int, _vp]),
@@ -301,6 +303,14 @@ class Column:
     def decode_schedule(self) -> int:
         v = C.c_int32()
         _check(lib().glc_column_get_decode_schedule(self._h, C.byref(v)))
+        return v.value
+
+    def set_decode_pace(self, pace: int) -> None:
+        _check(lib().glc_column_set_decode_pace(self._h, pace))
+
+    def decode_pace(self) -> int:
+        v = C.c_int32()
+        _check(lib().glc_column_get_decode_pace(self._h, C.byref(v)))
         return v.value
 
     def autotune_decode(self, out=None, reps: int = 10, stream=None):

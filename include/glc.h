@@ -171,10 +171,19 @@ enum {
 };
 int glc_column_set_decode_schedule(glc_column* col, int32_t schedule);
 int glc_column_get_decode_schedule(const glc_column* col, int32_t* schedule);
-/* Time every schedule on this column -- one warm-up launch each, then `reps`
- * rounds of one launch per schedule into d_out, each bracketed by HIP events
- * on `stream` -- and keep the one with the lowest median.  Synchronises the
- * stream.  schedule / ms (optional) receive the choice and its median time. */
+/* Pause each wave of a column's packed full decode makes after each row of
+ * stores, in units of 64 GPU cycles: 0 (default), 8, 16 or 24.  Spacing a
+ * wave's stores out can raise the whole decode's write rate (the int64 2^29
+ * timestamp column decodes ~6 % faster with 16); it changes the time only,
+ * never the output.  No counterpart in the reference (its decoders have no
+ * schedule to tune).  glc_column_autotune_decode picks it with the schedule. */
+int glc_column_set_decode_pace(glc_column* col, int32_t pace);
+int glc_column_get_decode_pace(const glc_column* col, int32_t* pace);
+/* Time every schedule with every pace on this column -- one warm-up launch
+ * each, then `reps` rounds of one launch per combination into d_out, each
+ * bracketed by HIP events on `stream` -- and keep the one with the lowest
+ * median.  Synchronises the stream.  schedule / ms (optional) receive the
+ * chosen schedule and its median time; glc_column_get_decode_pace the pace. */
 int glc_column_autotune_decode(glc_column* col, void* d_out, int32_t reps, void* stream, int32_t* schedule,
                                float* ms);
 /* decompressFullFile_Fix(cd, out, partition_size), G:768-788.  The column

@@ -15,7 +15,9 @@ These tests pin that rewrite on the CPU:
   * the whole fast conversion against the oracle's decode_pred (its own
     std::round + CUDA saturation) on random linear models whose runs pass the
     range check, including runs that end exactly on the range bounds and
-    unsigned runs that start below zero.
+    unsigned runs that start below zero; for 64-bit values both tiers (tier 1:
+    |p| <= 2^50 and the 1.5 * 2^52 constant, tier 2: p inside the type and
+    the plain conversion).
 A dropped copysign or 0.5 in place of 0.5 - 2^-54 fails the first (both
 are planted there); the second runs the conversion up to the range bounds.
 The GPU decoders themselves are checked on crafted columns that put tiles on
@@ -96,18 +98,24 @@ def fma(a: float, b: float, c: float) -> float:
 DTYPES = {O.DT_I32: (32, True), O.DT_U32: (32, False), O.DT_I64: (64, True), O.DT_U64: (64, False)}
 
 
-def fast_range(dt: int, plo: float, phi: float) -> bool:
+I64_HI = 9223372036854774784.0   # largest double below 2^63
+U64_HI = 18446744073709549568.0  # largest double below 2^64
+
+
+def fast_range(dt: int, plo: float, phi: float, tier: int = 1) -> bool:
     width, signed = DTYPES[dt]
     if width == 32 and signed:
         return all(-2147483647.0 <= p <= 2147483646.0 for p in (plo, phi))
     if width == 32:
         return plo <= 4294967294.0 and phi <= 4294967294.0
     if signed:
-        return all(-K50 <= p <= K50 for p in (plo, phi))
-    return plo <= K50 and phi <= K50
+        lo, hi = (-K50, K50) if tier == 1 else (-2.0 ** 63, I64_HI)
+        return all(lo <= p <= hi for p in (plo, phi))
+    hi = K50 if tier == 1 else U64_HI
+    return plo <= hi and phi <= hi
 
 
-def pred_fast(dt: int, t0: float, t1: float, i: int) -> int:
+def pred_fast(dt: int, t0: float, t1: float, i: int, tier: int = 1) -> int:
     width, signed = DTYPES[dt]
     p = fma(t1, float(i), t0)
     if signed:
@@ -115,18 +123,22 @@ def pred_fast(dt: int, t0: float, t1: float, i: int) -> int:
     else:
         p = max(0.0, p)
         r = math.trunc(p + C)
-    if width == 64:
+    if width == 64 and tier == 1:
         v = magic_i64(float(r))
     else:
-        v = int(r)
+        v = int(r)  # tier 2: the plain conversion of an in-range integral double
     return v & ((1 << width) - 1)
 
 
-def models(dt: int, rng):
+def models(dt: int, rng, tier: int = 1):
     """(t0, t1, i0, i1): runs of local indices, many at the range's edges."""
     width, signed = DTYPES[dt]
-    hi = K50 if width == 64 else (2147483646.0 if signed else 4294967294.0)
-    lo = -hi if signed else -1e6
+    if width == 64 and tier == 2:
+        hi = I64_HI if signed else U64_HI
+        lo = -2.0 ** 63 if signed else -1e6
+    else:
+        hi = K50 if width == 64 else (2147483646.0 if signed else 4294967294.0)
+        lo = -hi if signed else -1e6
     out = []
     for _ in range(120):
         n = int(rng.choice([128, 512, 1024]))
@@ -143,31 +155,35 @@ def models(dt: int, rng):
         elif kind == 3:  # random slopes and offsets across the range
             t1 = float(rng.uniform(-1e4, 1e4))
             t0 = float(rng.uniform(lo, hi)) / 2
+            if tier == 2 and rng.random() < 0.5:  # beyond tier 1: timestamp-like magnitudes
+                t0 = float(rng.choice([1.0, -1.0] if signed else [1.0])) * float(rng.uniform(2.0 ** 51, 2.0 ** 62))
         elif kind == 4:  # tiny slopes: fractions near 1/2
             t1 = float(rng.choice([1e-17, -1e-17, 2.0 ** -53, 3e-12]))
             t0 = float(rng.integers(-100, 100)) + float(rng.choice([0.5, C, 0.5 + 2.0 ** -52, -C]))
         else:  # just outside: must fail the check
             t1 = 1.0
             t0 = hi + 1.0 - t1 * (i0 + n - 1)
+            if width == 64 and tier == 2:  # +1 is lost at this magnitude: the next double up
+                t1, t0 = 0.0, float(np.nextafter(hi, np.inf))
         out.append((t0, t1, i0, i0 + n - 1))
     return out
 
 
-@pytest.mark.parametrize("dt", sorted(DTYPES))
-def test_fast_prediction_equals_oracle(dt):
-    rng = np.random.default_rng(100 + dt)
+@pytest.mark.parametrize("dt,tier", [(d, 1) for d in sorted(DTYPES)] + [(O.DT_I64, 2), (O.DT_U64, 2)])
+def test_fast_prediction_equals_oracle(dt, tier):
+    rng = np.random.default_rng(100 + dt + 10 * tier)
     checked = 0
     rejected = 0
-    for t0, t1, i0, i1 in models(dt, rng):
+    for t0, t1, i0, i1 in models(dt, rng, tier):
         plo, phi = fma(t1, float(i0), t0), fma(t1, float(i1), t0)
-        if not fast_range(dt, plo, phi):
+        if not fast_range(dt, plo, phi, tier):
             rejected += 1
             continue
         idx = sorted(set([i0, i1, i0 + 1, i1 - 1] + [int(x) for x in rng.integers(i0, i1 + 1, 24)]))
         prm = np.array([t0, t1, 0.0, 0.0])
         for i in idx:
             want = O.decode_pred(dt, O.MODEL_LINEAR, prm, i)
-            assert pred_fast(dt, t0, t1, i) == want, (dt, t0, t1, i)
+            assert pred_fast(dt, t0, t1, i, tier) == want, (dt, tier, t0, t1, i)
             checked += 1
     assert checked > 1500 and rejected > 0
     # NaN parameters never pass the check

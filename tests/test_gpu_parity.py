@@ -359,8 +359,13 @@ def crafted_models(dt):
          (hi - 3 * 4095.0, 3.0), (float("nan"), 1.0), (1.0, float("inf")), (123456.75, 0.125), (7.0, -0.5)]
     if signed:
         m += [(-hi + 2048.5, -1.0), (-hi, 0.0), (-hi - 0.5, 0.0), (-hi + 1.0, -1.0 / 4095)]
-    if width == 64:
-        m += [(2.0 ** 51 - 100.0, 0.0), (2.0 ** 50 + 0.25, 0.0), (2.0 ** 50 - 0.25, 0.0)]
+    if width == 64:  # beyond 2^50: the plain-conversion tier, and past the type: saturation
+        m += [(2.0 ** 51 - 100.0, 0.0), (2.0 ** 50 + 0.25, 0.0), (2.0 ** 50 - 0.25, 0.0),
+              (1.7e18 + 0.5, 977.25), (2.0 ** 50 - 2048.0, 1.0), (2.0 ** 62, -3.5)]
+        top = 9223372036854774784.0 if signed else 18446744073709549568.0  # largest doubles below 2^63 / 2^64
+        m += [(top, 0.0), (top - 4096.0 * 2048.0, 4096.0), (float(np.nextafter(top, np.inf)), 0.0)]
+        if signed:
+            m += [(-2.0 ** 63, 0.0), (-2.0 ** 63 + 2048.0 * 4096.0, -4096.0), (-1.7e18 - 0.5, -977.25)]
     return m
 
 
@@ -540,15 +545,21 @@ def test_decode_schedules_and_autotune(gpu, gen):
         for sch in (glc.DECODE_TILE_W1, glc.DECODE_TILE_W4, glc.DECODE_HALF_TILE_W1):
             col.set_decode_schedule(sch)
             assert col.decode_schedule() == sch
-            for mode in (glc.FULL_BITPACKED, glc.FULL_WORKSTEAL, glc.FULL_AUTO):
-                assert np.array_equal(host(col.decompress_full(mode)), v), (part, sch, mode)
+            for pace in (0, 8, 16, 24):
+                col.set_decode_pace(pace)
+                assert col.decode_pace() == pace
+                for mode in (glc.FULL_BITPACKED, glc.FULL_WORKSTEAL, glc.FULL_AUTO):
+                    assert np.array_equal(host(col.decompress_full(mode)), v), (part, sch, pace, mode)
+            col.set_decode_pace(0)
             # unaligned output: the one-value-per-lane path
             buf = torch.empty(n + 1, dtype=to_dev(v[:1]).dtype, device="cuda")
             col.decompress_full(glc.FULL_BITPACKED, out=buf[1:])
             assert np.array_equal(host(buf[1:]), v), (part, sch, "unaligned")
         sch, ms = col.autotune_decode(reps=3)
         assert sch in (glc.DECODE_TILE_W1, glc.DECODE_TILE_W4, glc.DECODE_HALF_TILE_W1) and ms > 0
-        assert col.decode_schedule() == sch
+        assert col.decode_schedule() == sch and col.decode_pace() in (0, 8, 16, 24)
         assert np.array_equal(host(col.decompress_full()), v)
     with pytest.raises(glc.GlcError):
         col.set_decode_schedule(7)
+    with pytest.raises(glc.GlcError):
+        col.set_decode_pace(5)
