@@ -41,6 +41,17 @@ SCHEDULE_KERNEL = {0: "k_decode_wave<{T}, packed, 1 wave/workgroup>", 1: "k_deco
                    2: "k_decode_half<{T}>"}
 
 
+def pick_schedule(col, out, args):
+    """The column's decode schedule and pace: the autotuner's choice, or the
+    ones given on the command line (--decode-schedule / --decode-pace, to
+    profile one configuration without the tuner's launches)."""
+    if args.decode_schedule is not None:
+        col.set_decode_schedule(args.decode_schedule)
+        col.set_decode_pace(args.decode_pace)
+        return args.decode_schedule, float("nan")
+    return col.autotune_decode(out, reps=10)
+
+
 def decode_traffic(workload: str, sched: int):
     """HBM bytes per decode launch of `workload` with schedule `sched`, from
     profiles/decode_traffic.json: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
@@ -293,7 +304,7 @@ def bench_i32(args):
     torch.cuda.synchronize()
     inf = col.info()
     out = torch.empty_like(x)
-    sched, _ = col.autotune_decode(out, reps=10)
+    sched, _ = pick_schedule(col, out, args)
     for _ in range(args.warmup):
         col.decompress_full(glc.FULL_BITPACKED, out=out)
     torch.cuda.synchronize()
@@ -356,7 +367,7 @@ def bench_c3(args):
         torch.cuda.synchronize()
         return sum(a.elapsed_time(c) for a, c in ev) / steps
 
-    sched, _ = col.autotune_decode(out, reps=10)
+    sched, _ = pick_schedule(col, out, args)
     res = {}
     res["packed_full_ms"] = timed(lambda: col.decompress_full(glc.FULL_BITPACKED, out=out), args.steps)
     assert torch.equal(out, x)
@@ -783,6 +794,9 @@ def main():
     ap.add_argument("--c5-budget-gib", type=float, default=16.0)
     ap.add_argument("--c5-max-inflight", type=int, default=256)
     ap.add_argument("--c5-clients", type=int, default=4, help="native threads submitting the served rows' requests")
+    ap.add_argument("--decode-schedule", type=int, default=None, choices=[0, 1, 2],
+                    help="c2/c3/i32: use this decode schedule (and --decode-pace) instead of the autotuner's choice")
+    ap.add_argument("--decode-pace", type=int, default=0, choices=[0, 8, 16, 24])
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain device copy of the input (known bytes) for counter calibration")
     ap.add_argument("--c4-columns", type=int, default=10_000)
@@ -845,7 +859,7 @@ def main():
 
     # the column's decode schedule, measured on this column (setup, like the
     # batched form's autotuner)
-    sched, sched_ms = col.autotune_decode(out, reps=10)
+    sched, sched_ms = pick_schedule(col, out, args)
     log(f"decode schedule {SCHEDULE_NAMES[sched]}, pace {col.decode_pace()} ({sched_ms:.4f} ms median in the tuner)")
     for _ in range(args.warmup):
         col.decompress_full(mode, out=out)
