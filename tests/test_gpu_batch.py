@@ -116,6 +116,32 @@ def test_batch_every_schedule_and_graph_replays(gpu):
         assert np.array_equal(single.payload(), b.column(j).payload())
 
 
+def test_batch_unaligned_outputs(gpu):
+    # outputs that are not 16-byte aligned take the general per-lane path; a
+    # batch with any such output decodes every column that way, under every
+    # schedule, and an all-aligned set of outputs afterwards takes the flat
+    # narrow path again
+    specs = W.c4_specs(num_columns=120, total=1 << 21)
+    dev = [W.c4_column_torch(s, index=j) for j, s in enumerate(specs)]
+    b = glc.Batch(dev, 1024)
+    b.compress()
+    backing = [torch.empty(t.numel() + 1, dtype=t.dtype, device="cuda") for t in dev]
+    outs = [bk[1:] if j % 3 == 0 else bk[:-1] for j, bk in enumerate(backing)]  # every third one unaligned
+    for w, t in glc.BATCH_SCHEDULES:
+        b.set_schedule(w, t)
+        for o in outs:
+            o.fill_(-1)
+        b.decompress(outs)
+        torch.cuda.synchronize()
+        for j, (x, o) in enumerate(zip(dev, outs)):
+            assert torch.equal(x, o), (w, t, j)
+    aligned = [torch.full_like(t, -1) for t in dev]
+    b.decompress(aligned)
+    torch.cuda.synchronize()
+    assert all(torch.equal(x, o) for x, o in zip(dev, aligned))
+    b.close()
+
+
 def test_batch_autotune_picks_a_supported_schedule(gpu):
     specs = W.c4_specs(num_columns=64, total=1 << 21)
     dev = [W.c4_column_torch(s, index=j) for j, s in enumerate(specs)]
