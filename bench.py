@@ -796,7 +796,7 @@ def main():
     ap.add_argument("--c5-clients", type=int, default=4, help="native threads submitting the served rows' requests")
     ap.add_argument("--decode-schedule", type=int, default=None, choices=[0, 1, 2],
                     help="c2/c3/i32: use this decode schedule (and --decode-pace) instead of the autotuner's choice")
-    ap.add_argument("--decode-pace", type=int, default=0, choices=[0, 8, 16, 24])
+    ap.add_argument("--decode-pace", type=int, default=0, choices=[0, 4, 8, 16, 24])
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain device copy of the input (known bytes) for counter calibration")
     ap.add_argument("--c4-columns", type=int, default=10_000)

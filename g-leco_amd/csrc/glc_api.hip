@@ -703,8 +703,8 @@ int glc_column_get_decode_schedule(const glc_column* c, int32_t* schedule) {
 
 int glc_column_set_decode_pace(glc_column* c, int32_t pace) {
     if (!c) return fail(GLC_ERR_INVALID_ARG, "NULL column");
-    if (pace != 0 && pace != 8 && pace != 16 && pace != 24)
-        return fail(GLC_ERR_INVALID_ARG, "decode pace must be 0, 8, 16 or 24");
+    if (pace != 0 && pace != 4 && pace != 8 && pace != 16 && pace != 24)
+        return fail(GLC_ERR_INVALID_ARG, "decode pace must be 0, 4, 8, 16 or 24");
     c->dec_pace = pace;
     return GLC_OK;
 }

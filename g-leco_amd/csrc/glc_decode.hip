@@ -253,7 +253,8 @@ __device__ __forceinline__ void wave_narrow(const DecArgs& a, const TileInfo& x,
 // column and the int32 column are fastest without.  The autotuner picks it
 // with the schedule.
 __device__ __forceinline__ void pace_row(int32_t pace) {
-    if (pace == 8) __builtin_amdgcn_s_sleep(8);
+    if (pace == 4) __builtin_amdgcn_s_sleep(4);
+    else if (pace == 8) __builtin_amdgcn_s_sleep(8);
     else if (pace == 16) __builtin_amdgcn_s_sleep(16);
     else if (pace == 24) __builtin_amdgcn_s_sleep(24);
 }
@@ -939,9 +940,9 @@ int glc_autotune_decode(glc_column* c, void* d_out, int32_t reps, hipStream_t s,
         return GLC_OK;
     }
     // every schedule with every pace
-    constexpr int kS = 3, kPaces = 4, kC = kS * kPaces;
+    constexpr int kS = 3, kPaces = 5, kC = kS * kPaces;
     const int sched_of[kS] = {GLC_DECODE_TILE_W1, GLC_DECODE_TILE_W4, GLC_DECODE_HALF_TILE_W1};
-    const int pace_of[kPaces] = {0, 8, 16, 24};
+    const int pace_of[kPaces] = {0, 4, 8, 16, 24};
     const int saved = c->dec_sched, saved_pace = c->dec_pace;
     std::vector<hipEvent_t> ev(static_cast<size_t>(2 * kC * reps), nullptr);
     int rc = GLC_OK;

@@ -172,7 +172,7 @@ enum {
 int glc_column_set_decode_schedule(glc_column* col, int32_t schedule);
 int glc_column_get_decode_schedule(const glc_column* col, int32_t* schedule);
 /* Pause each wave of a column's packed full decode makes after each row of
- * stores, in units of 64 GPU cycles: 0 (default), 8, 16 or 24.  Spacing a
+ * stores, in units of 64 GPU cycles: 0 (default), 4, 8, 16 or 24.  Spacing a
  * wave's stores out can raise the whole decode's write rate (the int64 2^29
  * timestamp column decodes ~6 % faster with 16); it changes the time only,
  * never the output.  No counterpart in the reference (its decoders have no

@@ -545,7 +545,7 @@ def test_decode_schedules_and_autotune(gpu, gen):
         for sch in (glc.DECODE_TILE_W1, glc.DECODE_TILE_W4, glc.DECODE_HALF_TILE_W1):
             col.set_decode_schedule(sch)
             assert col.decode_schedule() == sch
-            for pace in (0, 8, 16, 24):
+            for pace in (0, 4, 8, 16, 24):
                 col.set_decode_pace(pace)
                 assert col.decode_pace() == pace
                 for mode in (glc.FULL_BITPACKED, glc.FULL_WORKSTEAL, glc.FULL_AUTO):
@@ -557,7 +557,7 @@ def test_decode_schedules_and_autotune(gpu, gen):
             assert np.array_equal(host(buf[1:]), v), (part, sch, "unaligned")
         sch, ms = col.autotune_decode(reps=3)
         assert sch in (glc.DECODE_TILE_W1, glc.DECODE_TILE_W4, glc.DECODE_HALF_TILE_W1) and ms > 0
-        assert col.decode_schedule() == sch and col.decode_pace() in (0, 8, 16, 24)
+        assert col.decode_schedule() == sch and col.decode_pace() in (0, 4, 8, 16, 24)
         assert np.array_equal(host(col.decompress_full()), v)
     with pytest.raises(glc.GlcError):
         col.set_decode_schedule(7)
