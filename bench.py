@@ -774,6 +774,56 @@ def bench_e2e(args):
     print(json.dumps(line), flush=True)
 
 
+def hbm_probe(col, out, mode, inf, n):
+    """What the memory sustains for the C2 decode's traffic without decoding:
+    libglc_probe.so's kernel writes 4 KiB per one-wave workgroup (the half
+    tile) after loading that half tile's share of the compressed bytes, and a
+    plain hipMemsetAsync fills the same 2 GiB.  The decode is launched back to
+    back the same number of times for the comparison."""
+    import ctypes
+    import torch
+    lib = ctypes.CDLL(os.path.join(ROOT, "g-leco_amd", "lib", "libglc_probe.so"))
+    lib.glc_probe_traffic.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_size_t,
+                                      ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]
+    lib.glc_probe_memset.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]
+    reps = 30
+    out_bytes = n * 8
+    read_total = (inf.total_bits + 7) // 8 + META_BYTES_PER_PARTITION * inf.num_partitions
+    waves = out_bytes // 4096
+    rd = int(round(read_total / waves / 16)) * 16
+    src = torch.full((waves * max(rd, 512) // 8,), 3, dtype=torch.int64, device=out.device)
+    torch.cuda.synchronize()
+    res = {"reps": reps, "write_bytes": out_bytes}
+    ms = ctypes.c_float()
+    if lib.glc_probe_memset(ctypes.c_void_p(out.data_ptr()), out_bytes, reps, ctypes.byref(ms)) != 0:
+        raise RuntimeError("glc_probe_memset failed")
+    res["memset_ms"] = round(ms.value, 4)
+    res["memset_frac"] = round(out_bytes / (ms.value * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+    shapes = {}
+    for r in sorted({rd, 512}):
+        if lib.glc_probe_traffic(ctypes.c_void_p(src.data_ptr()), src.numel() * 8, ctypes.c_void_p(out.data_ptr()),
+                                 out_bytes, 4, r, reps, ctypes.byref(ms)) != 0:
+            raise RuntimeError("glc_probe_traffic failed")
+        shapes[str(r)] = {"ms": round(ms.value, 4),
+                          "frac": round((out_bytes + waves * r) / (ms.value * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    res["traffic_4KiB_per_wave_by_read_bytes"] = shapes
+    res["decode_read_bytes_per_4KiB"] = rd
+    torch.cuda.synchronize()
+    for _ in range(5):
+        col.decompress_full(mode, out=out)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        col.decompress_full(mode, out=out)
+    e1.record()
+    e1.synchronize()
+    dec = e0.elapsed_time(e1) / reps
+    res["decode_back_to_back_ms"] = round(dec, 4)
+    res["decode_vs_probe"] = round(shapes[str(rd)]["ms"] / dec, 4)
+    del src
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -799,6 +849,9 @@ def main():
     ap.add_argument("--decode-pace", type=int, default=0, choices=[0, 4, 8, 16, 24])
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain device copy of the input (known bytes) for counter calibration")
+    ap.add_argument("--hbm-probe", action="store_true",
+                    help="c2: also time what the memory sustains for the decode's traffic with no decoding "
+                         "(libglc_probe.so) and a plain 2-GiB fill, beside the decode launched back to back")
     ap.add_argument("--c4-columns", type=int, default=10_000)
     ap.add_argument("--sm-segments", default="",
                     help="splitmerge workload: comma list of merge segment lengths to time (boundaries must not change)")
@@ -902,6 +955,8 @@ def main():
         torch.cuda.synchronize()
         del y
 
+    probe = hbm_probe(col, out, mode, inf, n) if args.hbm_probe else None
+
     # per-request lookup latency: 1000 requests of 2^10 resident positions,
     # one at a time (launch, then wait), host wall clock per request; plus the
     # device time of each request's kernel (HIP events on the launch stream)
@@ -983,6 +1038,8 @@ def main():
                      "algorithmic_bytes": alg_bytes},
         "cpu_baseline": cpu,
     }
+    if probe is not None:
+        line["hbm_probe"] = probe
     print(json.dumps(line), flush=True)
 
 

@@ -36,6 +36,22 @@ def test_library_loads_and_exports_every_declared_symbol():
     assert b"gfx950" in L.glc_version()
 
 
+def test_probe_library_exports_and_rejects_bad_shapes():
+    # the benchmark's traffic probe (not part of libglc): argument checks run on the host
+    P = C.CDLL(os.path.join(ROOT, "g-leco_amd", "lib", "libglc_probe.so"))
+    P.glc_probe_traffic.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_int, C.c_int, C.c_int,
+                                    C.POINTER(C.c_float)]
+    P.glc_probe_memset.argtypes = [C.c_void_p, C.c_size_t, C.c_int, C.POINTER(C.c_float)]
+    ms = C.c_float()
+    fake = C.c_void_p(1 << 20)  # never dereferenced: every call below fails its checks first
+    assert P.glc_probe_traffic(None, 1 << 20, fake, 1 << 20, 4, 512, 1, C.byref(ms)) == -1
+    assert P.glc_probe_traffic(fake, 1 << 20, fake, 1 << 20, 3, 512, 1, C.byref(ms)) == -1    # 3 KiB per wave
+    assert P.glc_probe_traffic(fake, 1 << 20, fake, 1 << 20, 4, 500, 1, C.byref(ms)) == -1    # not 16-B slices
+    assert P.glc_probe_traffic(fake, 1 << 10, fake, 1 << 20, 4, 512, 1, C.byref(ms)) == -1    # input too small
+    assert P.glc_probe_traffic(fake, 1 << 20, fake, 1 << 20, 4, 512, 0, C.byref(ms)) == -1    # no repetitions
+    assert P.glc_probe_memset(None, 1 << 20, 1, C.byref(ms)) == -1
+
+
 def test_header_checksum_matches_oracle_and_reference():
     rng = np.random.default_rng(3)
     for n in [0, 1, 120, 4096]:
