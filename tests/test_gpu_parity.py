@@ -294,6 +294,35 @@ def test_parity_wide_and_direct_copy(gpu):
     assert ref.total_bits == 0
 
 
+@pytest.mark.parametrize("base", [2048, 512, 100, 4096])
+def test_parity_v6_widths_and_bases(gpu, base):
+    # V6 layouts with base <= 2048 take the fused compress (the fit writes each
+    # partition's residual codes to scratch, a copy pass places them); base
+    # 4096 (partitions up to 16384) and small bases (partitions that do not
+    # fill whole 1024-value tiles, first partitions below 256 values) cover
+    # its limits.  Same data as the fixed-partition width cases, plus the
+    # workload generators, ragged ends and tiny columns.
+    rng = np.random.default_rng(base)
+    n = 30_000 + 77
+    cases = [GENS[g](n) for g in sorted(GENS)]
+    u = W.c2_uint64(n, nseg=4)
+    u[4096:12288] += np.uint64(1 << 60)  # DIRECT_COPY partitions (64-bit fields)
+    cases.append(u)
+    w = rng.integers(-(1 << 62), 1 << 62, n).astype(np.int64)  # residuals of 33..64 bits
+    w[:8192] = rng.integers(-(1 << 31), 1 << 31, 8192)
+    cases.append(w)
+    cases.append(np.array([2**31 - 1, -2**31] * 4096, np.int32))  # 33-bit deltas
+    z = np.zeros(9000, np.int64)
+    z[::2] = (1 << 30) + 5
+    z[1::2] = -(1 << 30) - 5
+    cases.append(z)
+    cases.append(np.full(10_000, 7, np.uint32))  # 0-bit partitions, empty payload
+    for m in (1, 100, 1025, 2049):
+        cases.append((1000 + 9 * np.arange(m) + rng.integers(-40, 40, m)).astype(np.int64))
+    for v in cases:
+        full_case(v, "v6", base=base)
+
+
 def test_parity_explicit_partitions_with_gaps(gpu):
     v = W.c3_int64(30_000)
     st = np.array([0, 100, 5000, 5000, 9000, 20000], np.int32)
