@@ -774,18 +774,24 @@ def bench_e2e(args):
     print(json.dumps(line), flush=True)
 
 
-def hbm_probe(col, out, mode, inf, n):
+def hbm_probe(col, out, mode, inf, n, pos, look):
     """What the memory sustains for the C2 decode's traffic without decoding:
     libglc_probe.so's kernel writes 4 KiB per one-wave workgroup (the half
     tile) after loading that half tile's share of the compressed bytes, and a
     plain hipMemsetAsync fills the same 2 GiB.  The decode is launched back to
-    back the same number of times for the comparison."""
+    back the same number of times for the comparison.  Likewise for the
+    lookups: the probe's lookup kernel reads, for the same positions, a 32-B
+    tile record and two words of a payload of the column's size (fields of
+    the column's mean width), with no decoding."""
     import ctypes
     import torch
     lib = ctypes.CDLL(os.path.join(ROOT, "g-leco_amd", "lib", "libglc_probe.so"))
     lib.glc_probe_traffic.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_size_t,
                                       ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]
     lib.glc_probe_memset.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]
+    lib.glc_probe_lookup.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
+                                     ctypes.c_int64, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                                     ctypes.POINTER(ctypes.c_float)]
     reps = 30
     out_bytes = n * 8
     read_total = (inf.total_bits + 7) // 8 + META_BYTES_PER_PARTITION * inf.num_partitions
@@ -821,6 +827,32 @@ def hbm_probe(col, out, mode, inf, n):
     res["decode_back_to_back_ms"] = round(dec, 4)
     res["decode_vs_probe"] = round(shapes[str(rd)]["ms"] / dec, 4)
     del src
+    # lookups: records of the column's tiles, a payload of its size
+    ntiles = (n + 1023) // 1024
+    b = max(1, min(32, int(round(inf.total_bits / n))))
+    rec = torch.zeros(ntiles * 4, dtype=torch.int64, device=out.device)
+    rec[0::4] = torch.arange(ntiles, dtype=torch.int64, device=out.device) * (1024 * b)
+    nwords = (n * b + 31) // 32 + 2
+    words = torch.randint(0, 1 << 31, (nwords,), dtype=torch.int32, device=out.device)
+    res_look = torch.empty(pos.numel(), dtype=torch.int64, device=out.device)
+    torch.cuda.synchronize()
+    if lib.glc_probe_lookup(ctypes.c_void_p(rec.data_ptr()), ntiles, ctypes.c_void_p(words.data_ptr()), nwords,
+                            ctypes.c_void_p(pos.data_ptr()), pos.numel(), b, ctypes.c_void_p(res_look.data_ptr()),
+                            reps, ctypes.byref(ms)) != 0:
+        raise RuntimeError("glc_probe_lookup failed")
+    probe_look = ms.value
+    for _ in range(5):
+        col.random_access(pos, out=look)
+    e0.record()
+    for _ in range(reps):
+        col.random_access(pos, out=look)
+    e1.record()
+    e1.synchronize()
+    real_look = e0.elapsed_time(e1) / reps
+    res["lookup"] = {"positions": pos.numel(), "field_bits": b, "payload_bytes": nwords * 4,
+                     "probe_ms": round(probe_look, 4), "lookups_back_to_back_ms": round(real_look, 4),
+                     "lookups_vs_probe": round(probe_look / real_look, 4)}
+    del rec, words, res_look
     return res
 
 
@@ -955,7 +987,7 @@ def main():
         torch.cuda.synchronize()
         del y
 
-    probe = hbm_probe(col, out, mode, inf, n) if args.hbm_probe else None
+    probe = hbm_probe(col, out, mode, inf, n, pos, look) if args.hbm_probe else None
 
     # per-request lookup latency: 1000 requests of 2^10 resident positions,
     # one at a time (launch, then wait), host wall clock per request; plus the

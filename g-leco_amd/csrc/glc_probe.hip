@@ -2,6 +2,12 @@
 // full decode, with no decoding.  Measurement only; libglc.so does not contain
 // or call it.
 //
+// glc_probe_lookup: the batch lookup's memory pattern without the decoding:
+// per position a 32-byte record of its 1024-value tile (its first 8 bytes are
+// the tile's first bit), then the two payload words of a b-bit field, then an
+// 8-byte result; four lookups per thread with their loads staged, as the
+// library's lookup kernel does.
+//
 // glc_probe_traffic: each one-wave workgroup loads `read_bytes` contiguous bytes
 // of `in` with 16-byte lane loads (slices back to back), then writes `out_kib`
 // KiB of `out` with 16-byte non-temporal stores whose values depend on the
@@ -42,6 +48,47 @@ __global__ __launch_bounds__(64) void k_probe_traffic(const u32x4* __restrict__ 
             __builtin_nontemporal_store(u32x4{x + k, x ^ static_cast<unsigned>(k), static_cast<unsigned>(lane),
                                               static_cast<unsigned>(w)},
                                         out + o);
+    }
+}
+
+constexpr int kLookBatch = 4;
+constexpr int kLookThreads = 256;
+
+__global__ __launch_bounds__(kLookThreads) void k_probe_lookup(const int64_t* __restrict__ rec, int64_t ntiles,
+                                                               const uint32_t* __restrict__ words, int64_t nwords,
+                                                               const int32_t* __restrict__ pos, int64_t q, int b,
+                                                               uint64_t* __restrict__ out) {
+    const int64_t per_block = static_cast<int64_t>(kLookThreads) * kLookBatch;
+    const int64_t step = static_cast<int64_t>(gridDim.x) * per_block;
+    for (int64_t j0 = static_cast<int64_t>(blockIdx.x) * per_block + threadIdx.x; j0 < q; j0 += step) {
+        int64_t e[kLookBatch], w[kLookBatch];
+#pragma unroll
+        for (int k = 0; k < kLookBatch; ++k) {
+            const int64_t j = j0 + k * kLookThreads;
+            e[k] = (j < q) ? pos[j] : 0;
+        }
+#pragma unroll
+        for (int k = 0; k < kLookBatch; ++k) {
+            int64_t t = e[k] >> 10;
+            t = (t >= 0 && t < ntiles) ? t : 0;
+            w[k] = rec[4 * t] + (e[k] & 1023) * b;
+        }
+        uint32_t lo[kLookBatch], hi[kLookBatch];
+#pragma unroll
+        for (int k = 0; k < kLookBatch; ++k) {
+            int64_t wi = w[k] >> 5;
+            wi = (wi >= 0 && wi + 1 < nwords) ? wi : 0;
+            lo[k] = words[wi];
+            hi[k] = words[wi + 1];
+        }
+#pragma unroll
+        for (int k = 0; k < kLookBatch; ++k) {
+            const int64_t j = j0 + k * kLookThreads;
+            if (j < q) {
+                const uint64_t raw = ((static_cast<uint64_t>(hi[k]) << 32) | lo[k]) >> (w[k] & 31);
+                out[j] = raw & ((1ULL << b) - 1ULL);
+            }
+        }
     }
 }
 
@@ -89,6 +136,27 @@ int glc_probe_traffic(const void* d_in, size_t in_bytes, void* d_out, size_t out
             case 4: hipLaunchKernelGGL(k_probe_traffic<4>, g, t, 0, nullptr, in, in16, out, out16, rd16); break;
             default: hipLaunchKernelGGL(k_probe_traffic<8>, g, t, 0, nullptr, in, in16, out, out16, rd16); break;
         }
+        return hipGetLastError() == hipSuccess ? 0 : -1;
+    };
+    return time_launches(launch, reps, ms_out);
+}
+
+// d_rec: ntiles records of 32 bytes (first 8 bytes: the tile's first bit, an
+// int64); d_words: nwords >= 2 payload words; d_pos: q int32 positions
+// (outside [0, 1024 ntiles) they read tile 0; fields past the payload read
+// word 0); b: field width 1..32; d_out: q uint64 results.
+int glc_probe_lookup(const void* d_rec, int64_t ntiles, const uint32_t* d_words, int64_t nwords, const int32_t* d_pos,
+                     int64_t q, int b, void* d_out, int reps, float* ms_out) {
+    if (!d_rec || !d_words || !d_pos || !d_out || !ms_out || ntiles < 1 || nwords < 2 || q < 1 || b < 1 || b > 32 ||
+        reps < 1)
+        return -1;
+    const int64_t per_block = static_cast<int64_t>(kLookThreads) * kLookBatch;
+    int64_t g = (q + per_block - 1) / per_block;
+    if (g > 65536) g = 65536;
+    auto launch = [&]() -> int {
+        hipLaunchKernelGGL(k_probe_lookup, dim3(static_cast<unsigned>(g)), dim3(kLookThreads), 0, nullptr,
+                           static_cast<const int64_t*>(d_rec), ntiles, d_words, nwords, d_pos, q, b,
+                           static_cast<uint64_t*>(d_out));
         return hipGetLastError() == hipSuccess ? 0 : -1;
     };
     return time_launches(launch, reps, ms_out);

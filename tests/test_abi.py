@@ -50,6 +50,12 @@ def test_probe_library_exports_and_rejects_bad_shapes():
     assert P.glc_probe_traffic(fake, 1 << 10, fake, 1 << 20, 4, 512, 1, C.byref(ms)) == -1    # input too small
     assert P.glc_probe_traffic(fake, 1 << 20, fake, 1 << 20, 4, 512, 0, C.byref(ms)) == -1    # no repetitions
     assert P.glc_probe_memset(None, 1 << 20, 1, C.byref(ms)) == -1
+    P.glc_probe_lookup.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_int,
+                                   C.c_void_p, C.c_int, C.POINTER(C.c_float)]
+    assert P.glc_probe_lookup(fake, 16, fake, 100, fake, 10, 0, fake, 1, C.byref(ms)) == -1   # width 0
+    assert P.glc_probe_lookup(fake, 16, fake, 100, fake, 10, 33, fake, 1, C.byref(ms)) == -1  # width 33
+    assert P.glc_probe_lookup(fake, 0, fake, 100, fake, 10, 8, fake, 1, C.byref(ms)) == -1    # no tiles
+    assert P.glc_probe_lookup(None, 16, fake, 100, fake, 10, 8, fake, 1, C.byref(ms)) == -1
 
 
 def test_header_checksum_matches_oracle_and_reference():
