@@ -107,6 +107,68 @@ __device__ __forceinline__ T ra_lookup(const Desc* __restrict__ desc, const int3
 // tiles of the partition-spanning path take ra_lookup.
 constexpr int kRaBatch = 4;
 
+// Fixed-size partitions: the partition's 32-byte Desc by division, then its
+// payload words (or pre-unpacked delta), four lookups in flight per thread.
+// Elements outside [0, N) return 0; POLY2 partitions (decode-only blobs) go
+// through ra_lookup.
+template <class T, bool PLAIN>
+__device__ __forceinline__ void fixed_batch(const RaArgs& a, int64_t j0, const int64_t (&e)[kRaBatch]) {
+    T* __restrict__ out = static_cast<T*>(a.out);
+    bool in[kRaBatch];
+    Desc d[kRaBatch];
+#pragma unroll
+    for (int k = 0; k < kRaBatch; ++k) {
+        in[k] = e[k] >= 0 && e[k] < a.n;  // n < 2^31: 32-bit division
+        const int64_t p = in[k] ? static_cast<int64_t>(static_cast<uint32_t>(e[k]) / static_cast<uint32_t>(a.fixed)) : 0;
+        d[k] = a.desc[p];
+    }
+    uint32_t w0[kRaBatch], w1[kRaBatch], w2[kRaBatch];
+    int64_t pl[kRaBatch];
+#pragma unroll
+    for (int k = 0; k < kRaBatch; ++k) {
+        const int64_t kk = e[k] - d[k].start;
+        const bool ok = in[k] && kk >= 0 && kk < d[k].len;
+        if constexpr (PLAIN) {
+            pl[k] = a.plain[ok ? e[k] : 0];
+        } else {
+            const int b = desc_bits(d[k].packed);
+            const int64_t bp = ok ? desc_off(d[k].packed) + kk * b : 0;
+            const int64_t wi = bp >> 5;  // + 2 stays inside the payload's zero pad
+            w0[k] = a.words[wi];
+            w1[k] = a.words[wi + 1];
+            w2[k] = (b > 32) ? a.words[wi + 2] : 0u;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < kRaBatch; ++k) {
+        const int64_t j = j0 + k * kThreads;
+        if (j >= a.q) continue;
+        const int64_t kk = e[k] - d[k].start;
+        T v = T(0);
+        if (in[k] && kk >= 0 && kk < d[k].len) {
+            const int32_t type = desc_type(d[k].packed);
+            const int b = desc_bits(d[k].packed);
+            const int32_t i = static_cast<int32_t>(kk);
+            if (type == MODEL_POLY2) {
+                v = ra_lookup<T, PLAIN>(a.desc, a.tile_part, a.tinfo, a.words, a.plain, a.params, a.n, a.fixed, e[k]);
+            } else if constexpr (PLAIN) {
+                v = (type == MODEL_DIRECT_COPY) ? static_cast<T>(pl[k])
+                                                : apply_delta<T>(dec_pred<T>(d[k].theta0, d[k].theta1, i), pl[k]);
+            } else {
+                const uint32_t sh = static_cast<uint32_t>((desc_off(d[k].packed) + kk * b) & 31);
+                uint64_t raw = ((static_cast<uint64_t>(w1[k]) << 32) | w0[k]) >> sh;
+                if (sh + b > 64) raw |= static_cast<uint64_t>(w2[k]) << (64 - sh);
+                if (b < 64) raw &= (1ULL << b) - 1ULL;
+                if (b == 0) raw = 0;
+                v = (type == MODEL_DIRECT_COPY)
+                        ? static_cast<T>(raw)
+                        : apply_delta<T>(dec_pred<T>(d[k].theta0, d[k].theta1, i), sign_extend(raw, b));
+            }
+        }
+        out[j] = v;
+    }
+}
+
 template <class T, bool POS64, bool PLAIN>
 __global__ __launch_bounds__(kThreads) void k_random_access(RaArgs a) {
     T* __restrict__ out = static_cast<T*>(a.out);
@@ -121,9 +183,9 @@ __global__ __launch_bounds__(kThreads) void k_random_access(RaArgs a) {
                                       : static_cast<int64_t>(static_cast<const int32_t*>(a.pos)[j]))
                              : -1;
         }
-        // fixed-size lookups, empty columns and a released payload (all
-        // widths 0): one at a time through ra_lookup
-        if (a.fixed > 0 || a.n <= 0 || (!PLAIN && a.words == nullptr)) {
+        // empty columns and a released payload (all widths 0): one at a
+        // time through ra_lookup
+        if (a.n <= 0 || (!PLAIN && a.words == nullptr)) {
 #pragma unroll
             for (int k = 0; k < kRaBatch; ++k) {
                 const int64_t j = j0 + k * kThreads;
@@ -131,6 +193,10 @@ __global__ __launch_bounds__(kThreads) void k_random_access(RaArgs a) {
                     out[j] = ra_lookup<T, PLAIN>(a.desc, a.tile_part, a.tinfo, a.words, a.plain, a.params, a.n,
                                                  a.fixed, e[k]);
             }
+            continue;
+        }
+        if (a.fixed > 0) {  // O(1) partition p = pos / S (FixedPartitionKernels.cuh:76-77), staged the same way
+            fixed_batch<T, PLAIN>(a, j0, e);
             continue;
         }
         bool in[kRaBatch];
@@ -314,7 +380,12 @@ int launch_random_access(const glc_column* c, const void* d_pos, int pos64, int6
     a.n = c->n;
     a.P = c->P;
     a.n_index = c->n_index;
-    a.fixed = fixed_size;
+    // A column whose partitions are all [pS, (p+1)S) takes the O(1) partition
+    // index for generic lookups too: the same values, from one 32-byte Desc
+    // per partition instead of one 32-byte tile record per 1024 values (4x
+    // fewer record bytes at S = 4096: int64 2^29, 2^24 lookups 0.597 ->
+    // 0.493 ms).
+    a.fixed = fixed_size > 0 ? fixed_size : c->fixed_size;
     if (c->P == 0) a.n = 0;  // every query decodes to 0
     return dispatch_dtype(c->dtype, [&](auto t) {
         using T = decltype(t);
