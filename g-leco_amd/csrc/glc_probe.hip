@@ -5,7 +5,7 @@
 // glc_probe_lookup: the batch lookup's memory pattern without the decoding:
 // per position a 32-byte record of its 1024-value tile (its first 8 bytes are
 // the tile's first bit), then the two payload words of a b-bit field, then an
-// 8-byte result; four lookups per thread with their loads staged, as the
+// 8-byte result (non-temporal store); four lookups per thread with their loads staged, as the
 // library's lookup kernel does.
 //
 // glc_probe_traffic: each one-wave workgroup loads `read_bytes` contiguous bytes
@@ -86,7 +86,7 @@ __global__ __launch_bounds__(kLookThreads) void k_probe_lookup(const int64_t* __
             const int64_t j = j0 + k * kLookThreads;
             if (j < q) {
                 const uint64_t raw = ((static_cast<uint64_t>(hi[k]) << 32) | lo[k]) >> (w[k] & 31);
-                out[j] = raw & ((1ULL << b) - 1ULL);
+                __builtin_nontemporal_store(raw & ((1ULL << b) - 1ULL), out + j);  // as the library's lookups
             }
         }
     }
